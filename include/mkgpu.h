@@ -1,0 +1,213 @@
+/*
+ * mkgpu.h -- C ABI of libmkgpu.so, the MI355X (gfx950) kernels behind the
+ * minkasi mapmaking inner loop.
+ *
+ * Conventions (all entry points):
+ *   - Pointers named dev_* or documented as device memory are HIP device
+ *     pointers; every other pointer argument is host memory.
+ *   - Sizes are int64_t (the reference computes ndet*ndata in int, which
+ *     overflows past 2^31 samples; minkasi/lib/minkasi.c:52).
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *     Calls are asynchronous on that stream unless stated otherwise.
+ *   - The return value is a status (MK_OK == 0).  On failure,
+ *     mk_last_error() returns a message for the calling thread.  This is a
+ *     deliberate divergence from the reference, whose C functions return void
+ *     and check nothing.
+ *   - No entry point takes or returns a torch type.
+ */
+#ifndef MKGPU_H
+#define MKGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MK_OK 0
+#define MK_ERR_ARG 1     /* invalid argument (null pointer, bad size) */
+#define MK_ERR_HIP 2     /* HIP runtime error */
+#define MK_ERR_RANGE 3   /* index out of range (pixel outside the map) */
+#define MK_ERR_NUMERIC 4 /* non-finite value or zero curvature */
+#define MK_ERR_ALLOC 5   /* device allocation failed */
+
+/* Message for the last failing call on this thread. */
+const char *mk_last_error(void);
+/* Library ABI version (increments when a signature changes). */
+int mk_version(void);
+/* Number of bytes of workspace the reductions below need (device memory,
+ * 256-byte aligned, contents may be garbage on entry; the library leaves
+ * its counters at zero on exit so one workspace can be reused without
+ * clearing). */
+int64_t mk_reduce_workspace_bytes(void);
+
+/* ------------------------------------------------------------------ */
+/* Pointing -> pixel.
+ * Replaces SkyMap.pix_from_radec (minkasi/maps/skymap.py:457-490), which
+ * calls astropy wcs_world2pix for a CAR WCS built by get_wcs
+ * (minkasi/maps/utils.py:47-66), subtracts 1, rounds half-to-even and
+ * stores int32(x*ny + y).
+ * wcs10 (host, 10 doubles): euler[6] = {alpha_p, 90-delta_p, phi_p,
+ *   cos(90-delta_p), sin(90-delta_p), car_scale}, cdelt[2], crpix[2].
+ * ra, dec: device, radians, nsamp doubles each.  ipix: device int32 out.
+ * dev_nbad (device int64, may be NULL): number of samples whose pixel falls
+ * outside [0,nx)x[0,ny) is ADDED to it (the reference has no bounds check;
+ * those samples get the index the formula gives, as in the reference). */
+int mk_pix_from_radec_car(const double *ra, const double *dec, int64_t nsamp,
+                          const double *wcs10, int32_t nx, int32_t ny,
+                          int32_t *ipix, int64_t *dev_nbad, void *stream);
+/* Same rotation, raw 1-based pixel coordinates (wcs_world2pix(..., 1)),
+ * for the near-tie audit.  px, py: device doubles. */
+int mk_world2pix_car(const double *ra, const double *dec, int64_t nsamp,
+                     const double *wcs10, double *px, double *py, void *stream);
+/* Pointing of a separable raster, ra[d,t] = ra_b[t] + dra[d] and
+ * dec[d,t] = dec_b[t] + ddec[d], pixelized without materialising ra/dec
+ * (ndet x ndata samples, row-major).  Same arithmetic per sample as
+ * mk_pix_from_radec_car. */
+int mk_pix_from_raster_car(const double *ra_b, const double *dec_b,
+                           const double *dra, const double *ddec,
+                           int64_t ndet, int64_t ndata, const double *wcs10,
+                           int32_t nx, int32_t ny, int32_t *ipix,
+                           int64_t *dev_nbad, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* map2tod: dat[i] (+)= map[ipix[i]] for i < nsamp.
+ * Replaces map2tod_omp / map2tod_simple (minkasi/lib/minkasi.c:116-144)
+ * bound by minkasi/lib/minkasi.py (argtypes) and called from
+ * minkasi/mapmaking/map2tod.py:19-54.  nsamp = ndet*ndata of one TOD or of
+ * a whole batch of concatenated TODs.  Bit-exact (a gather). */
+int mk_map2tod_f32(float *dat, const float *map, int64_t nsamp,
+                   const int32_t *ipix, int do_add, void *stream);
+int mk_map2tod_f64(double *dat, const double *map, int64_t nsamp,
+                   const int32_t *ipix, int do_add, void *stream);
+
+/* tod2map with global atomics: map[ipix[i]] += dat[i].
+ * Counterpart of tod2map_atomic (minkasi.c:59-66); order-nondeterministic
+ * like the reference's tod2map_omp/tod2map_atomic.  Kept as the
+ * non-plan variant; the deterministic path is the plan below. */
+int mk_tod2map_atomic_f32(float *map, const float *dat, int64_t nsamp,
+                          const int32_t *ipix, void *stream);
+int mk_tod2map_atomic_f64(double *map, const double *dat, int64_t nsamp,
+                          const int32_t *ipix, void *stream);
+
+/* Deterministic tod2map.  Replaces tod2map_omp (minkasi.c:97-112, bound in
+ * minkasi/lib/minkasi.py, called from minkasi/mapmaking/tod2map.py:97-132
+ * via SkyMap.tod2map, minkasi/maps/skymap.py:569-609).
+ * A plan is built once per pixelization: samples are sorted by map tile
+ * (stable), every 64-sample group is ordered by pixel, and the work is cut
+ * into items.  Executing the plan sums each pixel's samples in a fixed
+ * order, so the map is bitwise reproducible.  The plan owns device memory
+ * allocated at creation; exec allocates nothing. */
+typedef struct mk_t2m_plan mk_t2m_plan;
+int mk_t2m_plan_create(const int32_t *ipix, int64_t nsamp, int32_t nx,
+                       int32_t ny, int32_t tile_x, int32_t tile_y,
+                       int32_t item_samples, mk_t2m_plan **plan,
+                       void *stream);
+/* map (+)= tod2map(dat).  accumulate=0 overwrites the map (pixels with no
+ * samples become 0), accumulate=1 adds (the reference's do_add=True). */
+int mk_t2m_plan_exec_f32(const mk_t2m_plan *plan, float *map, const float *dat,
+                         int accumulate, void *stream);
+int mk_t2m_plan_exec_f64(const mk_t2m_plan *plan, double *map,
+                         const double *dat, int accumulate, void *stream);
+/* info[0..7] = nsamp, nitems, ntiles_touched, tile_x, tile_y, plan bytes,
+ * partial-buffer bytes, max samples per item. */
+int mk_t2m_plan_info(const mk_t2m_plan *plan, int64_t *info);
+int mk_t2m_plan_destroy(mk_t2m_plan *plan);
+
+/* ------------------------------------------------------------------ */
+/* Map-space vector operations for PCG (minkasi/maps/skymap.py:428-440
+ * axpy, :651-666 dot, :724-740 __mul__; minkasi/maps/mapset.py:89-208).
+ * Maps are FP32; every reduction accumulates in FP64 with a fixed
+ * two-stage order (bitwise reproducible).  `work` is a device workspace of
+ * mk_reduce_workspace_bytes() bytes.  Device scalar block sc (FP64): */
+#define MK_SC_ZR 0     /* r.z */
+#define MK_SC_PAP 1    /* p.Ap */
+#define MK_SC_ALPHA 2  /* zr / pAp */
+#define MK_SC_BETA 3   /* zr_new / zr */
+#define MK_SC_RR 4     /* r.r */
+#define MK_SC_FLAG 5   /* nonzero: zero/negative curvature or non-finite */
+#define MK_SC_DOT 6    /* result of mk_dot_* */
+#define MK_SC_COUNT 8
+/* sc[MK_SC_DOT] = sum a*b (FP64 accumulation). */
+int mk_dot_f32(const float *a, const float *b, int64_t n, double *sc,
+               void *work, void *stream);
+int mk_dot_f64(const double *a, const double *b, int64_t n, double *sc,
+               void *work, void *stream);
+/* y += a*x with a a host scalar (SkyMap.axpy). */
+int mk_axpy_f32(float *y, const float *x, double a, int64_t n, void *stream);
+/* out = a*b elementwise (SkyMap.__mul__, Jacobi preconditioner). */
+int mk_mul_f32(float *out, const float *a, const float *b, int64_t n,
+               void *stream);
+/* PCG steps of minkasi/mapmaking/pcg.py:143-170 with device scalars:
+ *  mk_pcg_alpha:  pAp = p.Ap ; alpha = zr/pAp  (flag if pAp <= 0 / NaN)
+ *  mk_pcg_xr:     x += alpha p ; r -= alpha Ap ; rr = r.r
+ *  mk_pcg_xrz_jacobi: as mk_pcg_xr, then z = w*r and zr_new = r.z,
+ *                 beta = zr_new/zr, zr = zr_new (one fused pass)
+ *  mk_pcg_beta:   zr_new = r.z ; beta = zr_new/zr ; zr = zr_new
+ *  mk_pcg_p:      p = z + beta p
+ *  mk_pcg_init:   zr = r.z ; rr = r.r ; p = z                          */
+int mk_pcg_alpha(const float *p, const float *Ap, int64_t n, double *sc,
+                 void *work, void *stream);
+int mk_pcg_xr(float *x, float *r, const float *p, const float *Ap, int64_t n,
+              double *sc, void *work, void *stream);
+int mk_pcg_xrz_jacobi(float *x, float *r, float *z, const float *p,
+                      const float *Ap, const float *w, int64_t n, double *sc,
+                      void *work, void *stream);
+int mk_pcg_beta(const float *r, const float *z, int64_t n, double *sc,
+                void *work, void *stream);
+int mk_pcg_p(float *p, const float *z, int64_t n, const double *sc,
+             void *stream);
+int mk_pcg_init(const float *r, const float *z, float *p, int64_t n,
+                double *sc, void *work, void *stream);
+
+
+/* ------------------------------------------------------------------ */
+/* DCT-I (FFTW REDFT00, unnormalised): rows of length n >= 2.
+ * Replaces many_fft_r2r_1d / many_fftf_r2r_1d (minkasi/lib/mkfftw.c:174-254,
+ * bound in minkasi/lib/mkfftw.py, called by fft_r2r in
+ * minkasi/tools/fft.py:290-344; scipy fallback minkasi/tools/py_fft.py:239).
+ * Bluestein chirp-z on a power-of-two four-step FFT; any n.  x, y: device,
+ * nrow*n elements; y may equal x.  work: device workspace of at least
+ * mk_dct1_workspace_bytes(nrow, n, sizeof element) bytes, 256-B aligned.
+ * Synchronises the stream once (descriptor upload). */
+int64_t mk_dct1_workspace_bytes(int64_t nrow, int64_t n, int elem_bytes);
+int mk_dct1_f64(const double *x, double *y, int64_t nrow, int64_t n,
+                void *work, int64_t work_bytes, void *stream);
+int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
+                void *work, int64_t work_bytes, void *stream);
+
+/* Noise filter y = C W C x per detector mode, C = DCT-I, W = mywt: the
+ * DCT-I / weight / DCT-I core of NoiseSmoothedSVD.apply_noise
+ * (minkasi/mapmaking/noise.py:705-708), batched over every mode of every
+ * TOD.  Computed as one circular convolution per row through a real FFT of
+ * L = 2^q >= 3(n-1)+1 points (exact for any n).
+ * desc (host): 6 int64 per segment (TOD): rows, n, in_off, in_stride,
+ *   out_off, out_stride (element offsets into the exec in/out arrays).
+ * elem_bytes: 4 (FP32 transform) or 8 (FP64 transform).
+ * The plan owns its work buffer and spectra (allocated at creation). */
+typedef struct mk_filt_plan mk_filt_plan;
+int mk_filt_plan_create(int nseg, const int64_t *desc, int elem_bytes,
+                        mk_filt_plan **plan);
+/* wdct: device FP64, rows of C W (the DCT-I of each row of mywt), laid out
+ * exactly like the exec input (in_off/in_stride of desc). */
+int mk_filt_plan_set_weights(mk_filt_plan *plan, const double *wdct,
+                             void *stream);
+/* halve_edges=1 multiplies output samples 0 and n-1 by 1/2, which is the
+ * reference's dd[:,0] *= .5; dd[:,-1] *= .5 (noise.py:710-711) moved before
+ * the back-rotation it commutes with. */
+int mk_filt_plan_set_halve(mk_filt_plan *plan, int halve_edges);
+int mk_filt_plan_exec_f32(const mk_filt_plan *plan, const float *in,
+                          float *out, void *stream);
+int mk_filt_plan_exec_f64(const mk_filt_plan *plan, const double *in,
+                          double *out, void *stream);
+/* info[0..7] = nseg, K1/K3 items, K2 items, work bytes, spectrum bytes,
+ * elem_bytes, LDS bytes K1/K3, LDS bytes K2. */
+int mk_filt_plan_info(const mk_filt_plan *plan, int64_t *info);
+/* geom[4*i..4*i+3] = P1, P2, columns per item, P of segment i. */
+int mk_filt_plan_geometry(const mk_filt_plan *plan, int64_t *geom);
+int mk_filt_plan_destroy(mk_filt_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MKGPU_H */

@@ -1,0 +1,715 @@
+// Batched power-of-two four-step FFT engine and the two transforms built on it.
+//
+// NoiseSmoothedSVD.apply_noise (minkasi/mapmaking/noise.py:704-711) applies,
+// per rotated detector mode y (length n), y'' = C W C y where C is the
+// unnormalised DCT-I (FFTW REDFT00, minkasi/lib/mkfftw.c:174-212;
+// scipy.fft.dct type 1, minkasi/tools/py_fft.py:239) and W = mywt.  With
+// m = n-1 and N = 2m, C W C y = (even extension of y) circularly convolved
+// (period N) with h = C W (because C C = N I).  The engine evaluates that
+// circular convolution exactly as a linear one through a real FFT of
+// L = 2^q >= 3m+1 points: the input is the even periodic extension z'
+// on [-m, 2m], the kernel is h placed symmetrically (so its spectrum B is
+// real and even), and the outputs needed are the window [m, 2m].  This works
+// for any n (no Bluestein, no smooth-length requirement) and fuses the
+// forward transform, the weight multiply and the inverse transform.
+//
+// Stand-alone DCT-I (fft_r2r, minkasi/tools/fft.py:290-344) uses Bluestein's
+// chirp-z algorithm (Bluestein 1970; Rabiner, Schafer & Rader 1969) on the
+// same engine: Y_k = Re(chi_k sum_j c_j x_j chi_j conj(chi_{k-j})) with
+// chi_j = exp(-i pi j^2 / N), c_j = 1 at j = 0, m and 2 otherwise, the
+// convolution evaluated with a complex FFT of P = 2^q >= 2m+1 points.
+//
+// Four-step layout (P = P1*P2 complex points, j = P2*j1 + j2, k = k1 + P1*k2):
+//   K1: column FFTs of length P1 (one workgroup = cw adjacent columns, so
+//       global reads are contiguous runs), twiddle w_P^(j2*k1), store T[k1][j2].
+//   K2: row FFTs of length P2 (rows k1 and P1-k1 together, which is where the
+//       real-FFT unpacking partner C_(P-k) lives), pointwise spectral work,
+//       inverse row FFTs, conjugate twiddle, store back.
+//   K3: inverse column FFTs of length P1, 1/P scale, write the wanted outputs.
+// Every LDS sub-FFT is a radix-4/radix-2 Stockham pass sequence with a
+// per-workgroup twiddle table.  Each launch covers every row of every
+// segment (TOD) in the batch through a work list.
+#include "common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+
+__device__ __forceinline__ void sincospi_t(float x, float* s, float* c) { sincospif(x, s, c); }
+__device__ __forceinline__ void sincospi_t(double x, double* s, double* c) { sincospi(x, s, c); }
+
+// exp(-i*pi*num/den) for integers 0 <= num < 2*den (den a power of two
+// makes num/den exact).
+template <typename T>
+__device__ __forceinline__ cx<T> expmi(int64_t num, int64_t den) {
+  T s, c;
+  sincospi_t((T)(-(double)num / (double)den), &s, &c);
+  return {c, s};
+}
+
+constexpr int kThreads = 256;
+constexpr int kEPT = 32;                       // complex elements per thread in an LDS FFT
+constexpr int kMaxWG = kThreads * kEPT;        // 8192 complex points per workgroup
+constexpr int kMaxLds = 144 * 1024;            // dynamic LDS budget per workgroup (gfx950 has 160 KB)
+
+// In-place batched LDS FFT: buf[b*bstride + e*estride], e < len = 2^l2len,
+// b < nbatch, nbatch*len <= kMaxWG.  tw[i] = exp(-2 pi i i/len).
+template <typename T, bool INV>
+__device__ void lds_fft(cx<T>* buf, const cx<T>* tw, int l2len, int nbatch, int bstride, int estride) {
+  const int len = 1 << l2len;
+  const int tid = threadIdx.x;
+  int Ns = 1, lns = 0;
+  int rem = l2len;
+  while (rem > 0) {
+    if (rem >= 2) {
+      const int lq = l2len - 2, q = 1 << lq;
+      const int nbf = nbatch * q;
+      cx<T> v[kEPT];
+#pragma unroll
+      for (int u = 0; u < kEPT / 4; u++) {
+        int bf = tid + u * kThreads;
+        if (bf < nbf) {
+          int bt = bf >> lq, j = bf & (q - 1);
+          int k = j & (Ns - 1);
+          cx<T>* base = buf + bt * bstride;
+#pragma unroll
+          for (int r = 0; r < 4; r++) v[4 * u + r] = base[(j + r * q) * estride];
+          int st = k << (l2len - lns - 2);  // (len/(4 Ns)) * k
+#pragma unroll
+          for (int r = 1; r < 4; r++) {
+            cx<T> w = tw[st * r];
+            if (INV) w = cconj(w);
+            v[4 * u + r] = cmul(v[4 * u + r], w);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kEPT / 4; u++) {
+        int bf = tid + u * kThreads;
+        if (bf < nbf) {
+          int bt = bf >> lq, j = bf & (q - 1);
+          int k = j & (Ns - 1);
+          cx<T> a0 = cadd(v[4 * u], v[4 * u + 2]), a1 = csub(v[4 * u], v[4 * u + 2]);
+          cx<T> a2 = cadd(v[4 * u + 1], v[4 * u + 3]), d = csub(v[4 * u + 1], v[4 * u + 3]);
+          cx<T> a3 = INV ? cx<T>{-d.im, d.re} : cx<T>{d.im, -d.re};  // (+/-i) * d
+          cx<T>* base = buf + bt * bstride;
+          int idx = ((j >> lns) << (lns + 2)) + k;
+          base[idx * estride] = cadd(a0, a2);
+          base[(idx + Ns) * estride] = cadd(a1, a3);
+          base[(idx + 2 * Ns) * estride] = csub(a0, a2);
+          base[(idx + 3 * Ns) * estride] = csub(a1, a3);
+        }
+      }
+      __syncthreads();
+      Ns <<= 2; lns += 2; rem -= 2;
+    } else {
+      const int lq = l2len - 1, q = 1 << lq;
+      const int nbf = nbatch * q;
+      cx<T> v[kEPT];
+#pragma unroll
+      for (int u = 0; u < kEPT / 2; u++) {
+        int bf = tid + u * kThreads;
+        if (bf < nbf) {
+          int bt = bf >> lq, j = bf & (q - 1);
+          int k = j & (Ns - 1);
+          cx<T>* base = buf + bt * bstride;
+          v[2 * u] = base[j * estride];
+          cx<T> w = tw[k << (l2len - lns - 1)];
+          if (INV) w = cconj(w);
+          v[2 * u + 1] = cmul(base[(j + q) * estride], w);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kEPT / 2; u++) {
+        int bf = tid + u * kThreads;
+        if (bf < nbf) {
+          int bt = bf >> lq, j = bf & (q - 1);
+          int k = j & (Ns - 1);
+          cx<T>* base = buf + bt * bstride;
+          int idx = ((j >> lns) << (lns + 1)) + k;
+          base[idx * estride] = cadd(v[2 * u], v[2 * u + 1]);
+          base[(idx + Ns) * estride] = csub(v[2 * u], v[2 * u + 1]);
+        }
+      }
+      __syncthreads();
+      Ns <<= 1; lns += 1; rem -= 1;
+    }
+  }
+}
+
+template <typename T>
+__device__ void fill_twiddles(cx<T>* tw, int l2len) {
+  const int len = 1 << l2len;
+  for (int i = threadIdx.x; i < len; i += kThreads) tw[i] = expmi<T>(2 * (int64_t)i, len);
+  __syncthreads();
+}
+
+// Segment: `rows` rows sharing one length n and one plan.
+struct Seg {
+  int64_t in_off, in_stride;    // input rows (elements of the input type)
+  int64_t out_off, out_stride;  // output rows
+  int64_t t_off, t_stride;      // T buffer rows (complex)
+  int64_t s_off, s_stride;      // spectrum rows (filter: P+1 reals; Bluestein: P complex, stride 0)
+  int32_t rows, n, l1, l2, cw, flags;
+  int64_t base1, base2, base3;  // first global item of K1, K2, K3
+  int32_t per1, per2;           // items per row for K1/K3, K2
+};
+
+enum Mode { kFilter = 0, kFSpec = 1, kBlue = 2, kBSpec = 3 };
+
+__device__ inline int find_seg(const Seg* segs, int nseg, int64_t g, int which) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    int64_t b = which == 1 ? segs[mid].base1 : (which == 2 ? segs[mid].base2 : segs[mid].base3);
+    if (b <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---- K1: column FFTs -------------------------------------------------------
+template <typename T, typename TIn, int MODE>
+__global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ segs, int nseg,
+                                                      const TIn* __restrict__ in, cx<T>* __restrict__ tbuf) {
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 1);
+  const Seg sg = segs[s];
+  const int64_t local = (int64_t)blockIdx.x - sg.base1;
+  const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
+  const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
+  const int64_t P = (int64_t)P1 * P2;
+  const int64_t m = sg.n - 1, N = 2 * m;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [P1][cw]
+  cx<T>* tw = buf + P1 * cw;                          // [P1]
+  const TIn* x = in + sg.in_off + (int64_t)row * sg.in_stride;
+  const int j2_0 = jb * cw;
+  if (MODE == kFilter || MODE == kFSpec) {
+    // real input packed two per complex point: a_(2j), a_(2j+1)
+    T* rb = reinterpret_cast<T*>(buf);
+    const int64_t L = 2 * P;
+    for (int idx = threadIdx.x; idx < P1 * cw * 2; idx += kThreads) {
+      int j1 = idx / (2 * cw), w = idx - j1 * 2 * cw;
+      int64_t i = 2 * ((int64_t)P2 * j1 + j2_0) + w;  // a-index
+      T val = T(0);
+      if (MODE == kFilter) {
+        if (i <= 3 * m) {
+          int64_t t = i - m;
+          int64_t src = t < 0 ? -t : (t <= m ? t : 2 * m - t);
+          val = (T)x[src];
+        }
+      } else {
+        // symmetric embedding of h (rows of DCT-I(W)): b_i = h_i (i < m),
+        // h_m/2 at i = m and L-m, h_(L-i) for L-m < i < L
+        if (i < m) val = (T)x[i];
+        else if (i == m || i == L - m) val = (T)x[m] * T(0.5);
+        else if (i > L - m) val = (T)x[L - i];
+      }
+      rb[2 * (j1 * cw) + w] = val;  // buf[j1][w/2].(re|im)
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
+      int j1 = idx / cw, c = idx - j1 * cw;
+      int64_t j = (int64_t)P2 * j1 + j2_0 + c;
+      cx<T> val{T(0), T(0)};
+      if (MODE == kBlue) {
+        if (j <= m) {
+          T wgt = (j == 0 || j == m) ? T(1) : T(2);
+          cx<T> ch = expmi<T>((j * j) % (2 * N), N);
+          val = cscale(ch, wgt * (T)x[j]);
+        }
+      } else {  // kBSpec: conj(chirp) at lags -m..m, placed mod P
+        int64_t t = j <= m ? j : (j >= P - m ? P - j : -1);
+        if (t >= 0) val = cconj(expmi<T>((t * t) % (2 * N), N));
+      }
+      buf[j1 * cw + c] = val;
+    }
+  }
+  fill_twiddles<T>(tw, l1);  // has a barrier
+  lds_fft<T, false>(buf, tw, l1, cw, 1, cw);
+  cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
+    int k1 = idx / cw, c = idx - k1 * cw;
+    int64_t j2 = j2_0 + c;
+    cx<T> v = cmul(buf[k1 * cw + c], expmi<T>(2 * ((j2 * k1) & (P - 1)), P));
+    trow[(int64_t)k1 * P2 + j2] = v;
+  }
+}
+
+// ---- K2: row FFTs + spectral work + inverse row FFTs ----------------------
+// Real modes process row pair (k1, P1-k1); complex modes one row.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ segs, int nseg,
+                                                      cx<T>* __restrict__ tbuf, T* __restrict__ spec_r,
+                                                      cx<T>* __restrict__ spec_c) {
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 2);
+  const Seg sg = segs[s];
+  const int64_t local = (int64_t)blockIdx.x - sg.base2;
+  const int row = (int)(local / sg.per2), k1 = (int)(local % sg.per2);
+  const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2;
+  const int64_t P = (int64_t)P1 * P2;
+  cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);  // [2][P2]
+  cx<T>* Yb = X + 2 * P2;                          // [2][P2] (real modes)
+  cx<T>* tw = Yb + 2 * P2;                         // [P2]
+  const bool real_mode = (MODE == kFilter || MODE == kFSpec);
+  const int k1b = real_mode ? ((P1 - k1) & (P1 - 1)) : k1;
+  const bool pair = real_mode && (k1b != k1);
+  const int nrow = pair ? 2 : 1;
+  for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
+    int r = idx >> l2, k2 = idx & (P2 - 1);
+    int kk1 = r ? k1b : k1;
+    X[idx] = trow[(int64_t)kk1 * P2 + k2];
+  }
+  fill_twiddles<T>(tw, l2);
+  lds_fft<T, false>(X, tw, l2, nrow, P2, 1);
+  if (MODE == kBSpec) {
+    cx<T>* srow = spec_c + sg.s_off;
+    for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) srow[(int64_t)k1 * P2 + k2] = X[k2];
+    return;
+  }
+  if (MODE == kBlue) {
+    const cx<T>* srow = spec_c + sg.s_off;
+    for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) X[k2] = cmul(X[k2], srow[(int64_t)k1 * P2 + k2]);
+    __syncthreads();
+  } else {
+    // unpack the real spectrum A_k = E_k + w_L^k O_k for k = kk1 + P1*k2
+    const int64_t L = 2 * P;
+    T* srow = spec_r + sg.s_off + (int64_t)row * sg.s_stride;
+    T YPval = T(0);
+    for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
+      int r = idx >> l2, k2 = idx & (P2 - 1);
+      int kk1 = r ? k1b : k1;
+      // partner index P-k
+      int pr, pk2;
+      if (kk1 == 0) { pr = 0; pk2 = (P2 - k2) & (P2 - 1); }
+      else { pr = (kk1 == k1) ? (pair ? 1 : 0) : 0; pk2 = P2 - 1 - k2; }
+      cx<T> C = X[idx], Cp = cconj(X[pr * P2 + pk2]);
+      cx<T> E = cscale(cadd(C, Cp), T(0.5));
+      cx<T> D = csub(C, Cp);
+      cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};  // (C - conj Cp)/(2i)
+      int64_t k = kk1 + (int64_t)P1 * k2;
+      cx<T> A = cadd(E, cmul(expmi<T>(k, P), O));  // w_L^k = exp(-i pi k/P)
+      if (MODE == kFSpec) {
+        srow[(int64_t)kk1 * P2 + k2] = A.re;
+        if (k == 0) srow[P] = E.re - O.re;  // A_P
+      } else {
+        T b = srow[(int64_t)kk1 * P2 + k2];
+        Yb[idx] = cscale(A, b);
+        if (k == 0) YPval = (E.re - O.re) * srow[P];
+      }
+      (void)L;
+    }
+    if (MODE == kFSpec) return;
+    // Y_P is produced by the thread that owns k = 0; share it via LDS
+    __shared__ T yp_sh;
+    if (k1 == 0 && threadIdx.x == 0) yp_sh = YPval;
+    __syncthreads();
+    // pack for the inverse: D_k = E'_k + i O'_k
+    for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
+      int r = idx >> l2, k2 = idx & (P2 - 1);
+      int kk1 = r ? k1b : k1;
+      int64_t k = kk1 + (int64_t)P1 * k2;
+      cx<T> Y = Yb[idx], Ym;
+      if (k == 0) {
+        Ym = {yp_sh, T(0)};
+      } else {
+        int pr, pk2;
+        if (kk1 == 0) { pr = 0; pk2 = (P2 - k2) & (P2 - 1); }
+        else { pr = (kk1 == k1) ? (pair ? 1 : 0) : 0; pk2 = P2 - 1 - k2; }
+        Ym = cconj(Yb[pr * P2 + pk2]);
+      }
+      cx<T> Ep = cscale(cadd(Y, Ym), T(0.5));
+      cx<T> Op = cmul(cscale(csub(Y, Ym), T(0.5)), cconj(expmi<T>(k, P)));
+      X[idx] = {Ep.re - Op.im, Ep.im + Op.re};  // Ep + i*Op
+    }
+    __syncthreads();
+  }
+  lds_fft<T, true>(X, tw, l2, nrow, P2, 1);
+  for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
+    int r = idx >> l2, j2 = idx & (P2 - 1);
+    int kk1 = r ? k1b : k1;
+    cx<T> v = cmul(X[idx], cconj(expmi<T>(2 * (((int64_t)j2 * kk1) & (P - 1)), P)));
+    trow[(int64_t)kk1 * P2 + j2] = v;
+  }
+}
+
+// ---- K3: inverse column FFTs + output -------------------------------------
+template <typename T, typename TOut, int MODE>
+__global__ void __launch_bounds__(kThreads) k3_kernel(const Seg* __restrict__ segs, int nseg,
+                                                      const cx<T>* __restrict__ tbuf, TOut* __restrict__ out) {
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 3);
+  const Seg sg = segs[s];
+  const int64_t local = (int64_t)blockIdx.x - sg.base3;
+  const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
+  const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
+  const int64_t P = (int64_t)P1 * P2;
+  const int64_t m = sg.n - 1, N = 2 * m;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* tw = buf + P1 * cw;
+  const cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  const int j2_0 = jb * cw;
+  for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
+    int k1 = idx / cw, c = idx - k1 * cw;
+    buf[k1 * cw + c] = trow[(int64_t)k1 * P2 + j2_0 + c];
+  }
+  fill_twiddles<T>(tw, l1);
+  lds_fft<T, true>(buf, tw, l1, cw, 1, cw);
+  const T scale = T(1) / (T)P;
+  TOut* y = out + sg.out_off + (int64_t)row * sg.out_stride;
+  if (MODE == kFilter) {
+    const T* rb = reinterpret_cast<const T*>(buf);
+    const bool halve = sg.flags & 1;
+    for (int idx = threadIdx.x; idx < P1 * cw * 2; idx += kThreads) {
+      int j1 = idx / (2 * cw), w = idx - j1 * 2 * cw;
+      int64_t i = 2 * ((int64_t)P2 * j1 + j2_0) + w;
+      int64_t k = i - m;
+      if (k >= 0 && k <= m) {
+        T v = rb[2 * (j1 * cw) + w] * scale;
+        if (halve && (k == 0 || k == m)) v *= T(0.5);
+        y[k] = (TOut)v;
+      }
+    }
+  } else {  // kBlue
+    for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
+      int j1 = idx / cw, c = idx - j1 * cw;
+      int64_t k = (int64_t)P2 * j1 + j2_0 + c;
+      if (k <= m) {
+        cx<T> w = cscale(buf[j1 * cw + c], scale);
+        cx<T> ch = expmi<T>((k * k) % (2 * N), N);
+        y[k] = (TOut)(ch.re * w.re - ch.im * w.im);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side planning
+
+int log2_ceil(int64_t x) {
+  int q = 0;
+  while ((int64_t(1) << q) < x) q++;
+  return q;
+}
+
+struct Geometry {
+  int l1, l2, cw;
+  int64_t P;
+};
+
+// P = 2^q complex points split as P1 = 2^floor(q/2) <= P2.
+Geometry geometry(int64_t P_min, int elem_bytes) {
+  int q = std::max(2, log2_ceil(P_min));
+  Geometry g;
+  g.l1 = q / 2;
+  g.l2 = q - g.l1;
+  g.P = int64_t(1) << q;
+  int P1 = 1 << g.l1, P2 = 1 << g.l2;
+  int cw = std::min(16, std::max(1, kMaxWG / P1));
+  cw = std::min(cw, P2);
+  // keep (cw + 1) * P1 complex values within the LDS budget
+  while (cw > 1 && (int64_t)(cw + 1) * P1 * 2 * elem_bytes > kMaxLds) cw >>= 1;
+  g.cw = cw;
+  return g;
+}
+
+template <typename K>
+void allow_big_lds(K kernel) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kMaxLds);
+}
+
+void allow_lds_all() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  allow_big_lds(k1_kernel<float, float, kFilter>);
+  allow_big_lds(k1_kernel<double, double, kFilter>);
+  allow_big_lds(k1_kernel<float, double, kFSpec>);
+  allow_big_lds(k1_kernel<double, double, kFSpec>);
+  allow_big_lds(k1_kernel<float, float, kBlue>);
+  allow_big_lds(k1_kernel<double, double, kBlue>);
+  allow_big_lds(k1_kernel<float, float, kBSpec>);
+  allow_big_lds(k1_kernel<double, double, kBSpec>);
+  allow_big_lds(k2_kernel<float, kFilter>);
+  allow_big_lds(k2_kernel<double, kFilter>);
+  allow_big_lds(k2_kernel<float, kFSpec>);
+  allow_big_lds(k2_kernel<double, kFSpec>);
+  allow_big_lds(k2_kernel<float, kBlue>);
+  allow_big_lds(k2_kernel<double, kBlue>);
+  allow_big_lds(k2_kernel<float, kBSpec>);
+  allow_big_lds(k2_kernel<double, kBSpec>);
+  allow_big_lds(k3_kernel<float, float, kFilter>);
+  allow_big_lds(k3_kernel<double, double, kFilter>);
+  allow_big_lds(k3_kernel<float, float, kBlue>);
+  allow_big_lds(k3_kernel<double, double, kBlue>);
+  (void)hipGetLastError();  // an unsupported attribute must not poison later launch checks
+}
+
+size_t lds_k13(const Geometry& g, int eb) { return (size_t)((1 << g.l1) * g.cw + (1 << g.l1)) * 2 * eb; }
+size_t lds_k2(const Geometry& g, int eb) { return (size_t)(5 * (1 << g.l2)) * 2 * eb + 64; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Filter plan: per segment (TOD) rows x n, input/output row offsets.
+
+struct mk_filt_plan {
+  int nseg = 0;
+  int elem_bytes = 4;  // 4: FP32 spectra and T buffer, 8: FP64
+  std::vector<Seg> h_segs;
+  Seg* d_segs = nullptr;
+  int64_t n1 = 0, n2 = 0;  // total K1/K3 items, K2 items
+  int64_t t_elems = 0, s_elems = 0;
+  void* tbuf = nullptr;    // complex
+  void* spec = nullptr;    // real, P+1 per row
+  size_t lds1 = 0, lds2 = 0;
+  bool have_spec = false;
+};
+
+namespace {
+void free_filt(mk_filt_plan* p) {
+  if (!p) return;
+  if (p->d_segs) (void)hipFree(p->d_segs);
+  if (p->tbuf) (void)hipFree(p->tbuf);
+  if (p->spec) (void)hipFree(p->spec);
+  delete p;
+}
+
+// desc per segment: {rows, n, in_off, in_stride, out_off, out_stride}
+int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<Seg>& segs, int64_t* n1,
+               int64_t* n2, int64_t* t_elems, int64_t* s_elems, size_t* lds1, size_t* lds2) {
+  segs.clear();
+  int64_t b1 = 0, b2 = 0, toff = 0, soff = 0;
+  size_t l1max = 0, l2max = 0;
+  for (int i = 0; i < nseg; i++) {
+    const int64_t* d = desc + 6 * i;
+    int64_t rows = d[0], n = d[1];
+    MK_REQUIRE(rows >= 0 && n >= 2 && n < (int64_t(1) << 30), MK_ERR_ARG,
+               "segment %d: need rows >= 0 and 2 <= n < 2^30 (rows=%lld n=%lld)", i, (long long)rows,
+               (long long)n);
+    int64_t m = n - 1;
+    Geometry g = geometry(filter ? (3 * m + 2) / 2 : 2 * m + 1, eb);
+    MK_REQUIRE(g.l1 + g.l2 <= 24, MK_ERR_ARG, "segment %d: n=%lld too long", i, (long long)n);
+    Seg s{};
+    s.in_off = d[2]; s.in_stride = d[3]; s.out_off = d[4]; s.out_stride = d[5];
+    s.rows = (int32_t)rows; s.n = (int32_t)n; s.l1 = g.l1; s.l2 = g.l2; s.cw = g.cw; s.flags = 0;
+    s.t_off = toff; s.t_stride = g.P;
+    s.s_off = soff; s.s_stride = filter ? g.P + 1 : 0;
+    s.per1 = (int32_t)((int64_t(1) << g.l2) / g.cw);
+    s.per2 = filter ? (1 << (g.l1 - 1)) + 1 : (1 << g.l1);
+    s.base1 = b1; s.base2 = b2; s.base3 = b1;
+    b1 += rows * s.per1;
+    b2 += rows * s.per2;
+    toff += rows * g.P;
+    soff += filter ? rows * (g.P + 1) : g.P;
+    l1max = std::max(l1max, lds_k13(g, eb));
+    l2max = std::max(l2max, lds_k2(g, eb));
+    MK_REQUIRE(l1max <= (size_t)kMaxLds && l2max <= (size_t)kMaxLds, MK_ERR_ARG,
+               "segment %d: n=%lld needs more LDS than the %d-byte budget at %d-byte precision", i,
+               (long long)n, kMaxLds, eb);
+    segs.push_back(s);
+  }
+  *n1 = b1; *n2 = b2; *t_elems = toff; *s_elems = soff; *lds1 = l1max; *lds2 = l2max;
+  return MK_OK;
+}
+
+template <typename T>
+void launch_fspec(mk_filt_plan* p, const void* wdct, hipStream_t st) {
+  k1_kernel<T, double, kFSpec><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg, (const double*)wdct,
+                                                                           (cx<T>*)p->tbuf);
+  k2_kernel<T, kFSpec><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf,
+                                                                   (T*)p->spec, nullptr);
+}
+
+template <typename T, typename TIO>
+void launch_filter(const mk_filt_plan* p, const TIO* in, TIO* out, hipStream_t st) {
+  k1_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg, in, (cx<T>*)p->tbuf);
+  k2_kernel<T, kFilter><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf,
+                                                                    (T*)p->spec, nullptr);
+  k3_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg,
+                                                                         (const cx<T>*)p->tbuf, out);
+}
+
+}  // namespace
+
+extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes, mk_filt_plan** out) {
+  MK_REQUIRE(out, MK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  MK_REQUIRE(nseg >= 0 && (nseg == 0 || desc), MK_ERR_ARG, "bad segment table");
+  MK_REQUIRE(elem_bytes == 4 || elem_bytes == 8, MK_ERR_ARG, "elem_bytes must be 4 or 8");
+  allow_lds_all();
+  mk_filt_plan* p = new mk_filt_plan();
+  p->nseg = nseg;
+  p->elem_bytes = elem_bytes;
+  int st = build_segs(nseg, desc, elem_bytes, true, p->h_segs, &p->n1, &p->n2, &p->t_elems, &p->s_elems, &p->lds1,
+                      &p->lds2);
+  if (st) { free_filt(p); return st; }
+  MK_REQUIRE(p->n1 < INT32_MAX && p->n2 < INT32_MAX, (free_filt(p), MK_ERR_ARG), "too many work items");
+  if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
+      hipMalloc(&p->tbuf, std::max<int64_t>(1, p->t_elems) * 2 * elem_bytes) != hipSuccess ||
+      hipMalloc(&p->spec, std::max<int64_t>(1, p->s_elems) * elem_bytes) != hipSuccess) {
+    free_filt(p);
+    mk::set_error("filter plan: device allocation failed");
+    return MK_ERR_ALLOC;
+  }
+  if (nseg && hipMemcpy(p->d_segs, p->h_segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
+    free_filt(p);
+    mk::set_error("filter plan: upload failed");
+    return MK_ERR_HIP;
+  }
+  *out = p;
+  return MK_OK;
+}
+
+extern "C" int mk_filt_plan_destroy(mk_filt_plan* p) {
+  free_filt(p);
+  return MK_OK;
+}
+
+extern "C" int mk_filt_plan_info(const mk_filt_plan* p, int64_t* info) {
+  MK_REQUIRE(p && info, MK_ERR_ARG, "null pointer");
+  info[0] = p->nseg;
+  info[1] = p->n1;
+  info[2] = p->n2;
+  info[3] = p->t_elems * 2 * p->elem_bytes;
+  info[4] = p->s_elems * p->elem_bytes;
+  info[5] = p->elem_bytes;
+  info[6] = (int64_t)p->lds1;
+  info[7] = (int64_t)p->lds2;
+  return MK_OK;
+}
+
+// Per-segment FFT geometry: out[4*i..] = {P1, P2, cw, P} of segment i.
+extern "C" int mk_filt_plan_geometry(const mk_filt_plan* p, int64_t* out) {
+  MK_REQUIRE(p && out, MK_ERR_ARG, "null pointer");
+  for (int i = 0; i < p->nseg; i++) {
+    const Seg& s = p->h_segs[i];
+    out[4 * i] = int64_t(1) << s.l1;
+    out[4 * i + 1] = int64_t(1) << s.l2;
+    out[4 * i + 2] = s.cw;
+    out[4 * i + 3] = s.t_stride;
+  }
+  return MK_OK;
+}
+
+extern "C" int mk_filt_plan_set_weights(mk_filt_plan* p, const double* wdct, void* stream) {
+  MK_REQUIRE(p && (p->nseg == 0 || wdct), MK_ERR_ARG, "null argument");
+  if (p->n1 == 0) { p->have_spec = true; return MK_OK; }
+  hipStream_t st = mk::as_stream(stream);
+  if (p->elem_bytes == 4) launch_fspec<float>(p, wdct, st);
+  else launch_fspec<double>(p, wdct, st);
+  int r = mk::check_launch("filter spectrum");
+  if (r == MK_OK) p->have_spec = true;
+  return r;
+}
+
+extern "C" int mk_filt_plan_exec_f32(const mk_filt_plan* p, const float* in, float* out, void* stream) {
+  MK_REQUIRE(p && in && out, MK_ERR_ARG, "null argument");
+  MK_REQUIRE(p->have_spec, MK_ERR_ARG, "filter weights not set");
+  MK_REQUIRE(p->elem_bytes == 4, MK_ERR_ARG, "plan was created for FP64");
+  if (p->n1 == 0) return MK_OK;
+  launch_filter<float, float>(p, in, out, mk::as_stream(stream));
+  return mk::check_launch("filter exec f32");
+}
+
+extern "C" int mk_filt_plan_exec_f64(const mk_filt_plan* p, const double* in, double* out, void* stream) {
+  MK_REQUIRE(p && in && out, MK_ERR_ARG, "null argument");
+  MK_REQUIRE(p->have_spec, MK_ERR_ARG, "filter weights not set");
+  MK_REQUIRE(p->elem_bytes == 8, MK_ERR_ARG, "plan was created for FP32");
+  if (p->n1 == 0) return MK_OK;
+  launch_filter<double, double>(p, in, out, mk::as_stream(stream));
+  return mk::check_launch("filter exec f64");
+}
+
+extern "C" int mk_filt_plan_set_halve(mk_filt_plan* p, int halve_edges) {
+  MK_REQUIRE(p, MK_ERR_ARG, "null plan");
+  for (auto& s : p->h_segs) s.flags = halve_edges ? 1 : 0;
+  if (p->nseg && hipMemcpy(p->d_segs, p->h_segs.data(), p->nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
+    mk::set_error("filter plan: upload failed");
+    return MK_ERR_HIP;
+  }
+  return MK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone DCT-I (Bluestein), rows of one length n.
+
+extern "C" int64_t mk_dct1_workspace_bytes(int64_t nrow, int64_t n, int elem_bytes) {
+  if (n < 2 || nrow < 0 || (elem_bytes != 4 && elem_bytes != 8)) return -1;
+  Geometry g = geometry(2 * (n - 1) + 1, elem_bytes);
+  return (nrow * g.P + g.P) * 2 * elem_bytes + 4096;
+}
+
+namespace {
+template <typename T, typename TIO>
+int dct1_impl(const TIO* x, TIO* y, int64_t nrow, int64_t n, void* work, int64_t work_bytes, void* stream) {
+  MK_REQUIRE(nrow >= 0 && n >= 2, MK_ERR_ARG, "DCT-I needs n >= 2 (got %lld)", (long long)n);
+  if (nrow == 0) return MK_OK;
+  MK_REQUIRE(x && y && work, MK_ERR_ARG, "null argument");
+  int64_t need = mk_dct1_workspace_bytes(nrow, n, sizeof(T));
+  MK_REQUIRE(work_bytes >= need, MK_ERR_ARG, "workspace too small (%lld < %lld)", (long long)work_bytes,
+             (long long)need);
+  MK_REQUIRE(((uintptr_t)work & 255) == 0, MK_ERR_ARG, "workspace must be 256-byte aligned");
+  allow_lds_all();
+  hipStream_t st = mk::as_stream(stream);
+  int64_t desc[6] = {nrow, n, 0, n, 0, n};
+  std::vector<Seg> segs;
+  int64_t n1, n2, te, se;
+  size_t l1, l2;
+  int r = build_segs(1, desc, sizeof(T), false, segs, &n1, &n2, &te, &se, &l1, &l2);
+  if (r) return r;
+  MK_REQUIRE(n1 < INT32_MAX && n2 < INT32_MAX, MK_ERR_ARG, "too many rows");
+  // workspace: [segment descriptor (aligned)] [V spectrum P] [T buffer nrow*P]
+  unsigned char* w = (unsigned char*)work;
+  Seg* d_seg = (Seg*)w;
+  cx<T>* V = (cx<T>*)(w + 4096);
+  cx<T>* tb = V + segs[0].t_stride;
+  Seg s0 = segs[0];
+  s0.s_off = 0;
+  MK_HIP(hipMemcpyAsync(d_seg, &s0, sizeof(Seg), hipMemcpyHostToDevice, st));
+  MK_HIP(hipStreamSynchronize(st));
+  // spectrum of the conjugate chirp: a one-row pass (rows = 1) over the same geometry
+  Seg sv = s0;
+  sv.rows = 1;
+  Seg* d_sv = d_seg + 1;
+  MK_HIP(hipMemcpyAsync(d_sv, &sv, sizeof(Seg), hipMemcpyHostToDevice, st));
+  MK_HIP(hipStreamSynchronize(st));
+  k1_kernel<T, TIO, kBSpec><<<(unsigned)s0.per1, kThreads, l1, st>>>(d_sv, 1, x, V);
+  k2_kernel<T, kBSpec><<<(unsigned)s0.per2, kThreads, l2, st>>>(d_sv, 1, V, nullptr, V);
+  k1_kernel<T, TIO, kBlue><<<(unsigned)n1, kThreads, l1, st>>>(d_seg, 1, x, tb);
+  k2_kernel<T, kBlue><<<(unsigned)n2, kThreads, l2, st>>>(d_seg, 1, tb, nullptr, V);
+  k3_kernel<T, TIO, kBlue><<<(unsigned)n1, kThreads, l1, st>>>(d_seg, 1, tb, y);
+  return mk::check_launch("dct1");
+}
+}  // namespace
+
+extern "C" int mk_dct1_f64(const double* x, double* y, int64_t nrow, int64_t n, void* work, int64_t work_bytes,
+                           void* stream) {
+  return dct1_impl<double, double>(x, y, nrow, n, work, work_bytes, stream);
+}
+extern "C" int mk_dct1_f32(const float* x, float* y, int64_t nrow, int64_t n, void* work, int64_t work_bytes,
+                           void* stream) {
+  return dct1_impl<float, float>(x, y, nrow, n, work, work_bytes, stream);
+}

@@ -1,0 +1,478 @@
+// TOD <-> map projections.
+//
+//  map2tod : coalesced gather, dat[i] (+)= map[ipix[i]].  8 B/sample of HBM
+//            traffic (int32 index in, value out); the map (<= 16 MB FP32 at
+//            2048^2) stays in L2 / Infinity Cache.
+//  tod2map : deterministic plan.  Samples are stably sorted by map tile,
+//            each 64-sample group is ordered by pixel, and each work item
+//            (<= item_samples entries of one tile) reduces its samples into a
+//            wave-private LDS tile with a wave-level segmented scan (no
+//            atomics), sums the four wave tiles in a fixed order and writes a
+//            partial tile.  A second kernel sums each tile's partials in item
+//            order.  The map is therefore bitwise reproducible run to run.
+//            Per sample the exec reads a uint32 entry, a uint16 local pixel
+//            and the value (10 B/sample for FP32).
+#include "common.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// map2tod
+
+template <typename T>
+__global__ void __launch_bounds__(256) map2tod_kernel(T* __restrict__ dat, const T* __restrict__ map, int64_t n,
+                                                      const int32_t* __restrict__ ipix, int do_add) {
+  // 4 samples per thread per step; the tail is handled scalar.
+  int64_t n4 = n >> 2;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int4* ip4 = reinterpret_cast<const int4*>(ipix);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
+    int4 p = ip4[q];
+    T v0 = map[p.x], v1 = map[p.y], v2 = map[p.z], v3 = map[p.w];
+    T* d = dat + 4 * q;
+    if (do_add) {
+      v0 += d[0]; v1 += d[1]; v2 += d[2]; v3 += d[3];
+    }
+    d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3;
+  }
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    T v = map[ipix[i]];
+    dat[i] = do_add ? dat[i] + v : v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tod2map_atomic_kernel(T* map, const T* __restrict__ dat, int64_t n,
+                                                             const int32_t* __restrict__ ipix) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    atomicAdd(map + ipix[i], dat[i]);
+}
+
+template <typename T>
+int map2tod_impl(T* dat, const T* map, int64_t nsamp, const int32_t* ipix, int do_add, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(dat && map && ipix, MK_ERR_ARG, "null pointer");
+  MK_REQUIRE(((uintptr_t)ipix & 15) == 0 && ((uintptr_t)dat % (4 * sizeof(T))) == 0, MK_ERR_ARG,
+             "map2tod: ipix must be 16-byte aligned and dat aligned to 4 elements");
+  map2tod_kernel<T><<<mk::grid_for(mk::cdiv(nsamp, 4), 256, 256 * 16), 256, 0, mk::as_stream(stream)>>>(
+      dat, map, nsamp, ipix, do_add);
+  return mk::check_launch("map2tod_kernel");
+}
+
+template <typename T>
+int tod2map_atomic_impl(T* map, const T* dat, int64_t nsamp, const int32_t* ipix, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(dat && map && ipix, MK_ERR_ARG, "null pointer");
+  tod2map_atomic_kernel<T><<<mk::grid_for(nsamp, 256, 256 * 16), 256, 0, mk::as_stream(stream)>>>(map, dat, nsamp,
+                                                                                                   ipix);
+  return mk::check_launch("tod2map_atomic_kernel");
+}
+
+}  // namespace
+
+extern "C" int mk_map2tod_f32(float* dat, const float* map, int64_t nsamp, const int32_t* ipix, int do_add,
+                              void* stream) {
+  return map2tod_impl<float>(dat, map, nsamp, ipix, do_add, stream);
+}
+extern "C" int mk_map2tod_f64(double* dat, const double* map, int64_t nsamp, const int32_t* ipix, int do_add,
+                              void* stream) {
+  return map2tod_impl<double>(dat, map, nsamp, ipix, do_add, stream);
+}
+extern "C" int mk_tod2map_atomic_f32(float* map, const float* dat, int64_t nsamp, const int32_t* ipix,
+                                     void* stream) {
+  return tod2map_atomic_impl<float>(map, dat, nsamp, ipix, stream);
+}
+extern "C" int mk_tod2map_atomic_f64(double* map, const double* dat, int64_t nsamp, const int32_t* ipix,
+                                     void* stream) {
+  return tod2map_atomic_impl<double>(map, dat, nsamp, ipix, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic tod2map plan
+
+struct T2MItem {
+  int64_t seg_base;   // first sample of the segment the entries are relative to
+  int64_t begin;      // first entry (index into entry/lpix arrays)
+  int32_t count;      // number of entries
+  int32_t tile;       // tile id = tx * ntile_y + ty
+};
+
+struct mk_t2m_plan {
+  int64_t nsamp = 0;
+  int32_t nx = 0, ny = 0, tile_x = 0, tile_y = 0, ntile_x = 0, ntile_y = 0;
+  int32_t item_samples = 0;
+  int64_t nitems = 0, ntiles_touched = 0, max_item = 0;
+  uint32_t* entry = nullptr;     // [nsamp] sample offset within segment
+  uint16_t* lpix = nullptr;      // [nsamp] pixel within tile
+  T2MItem* items = nullptr;      // [nitems] sorted by (tile, segment, position)
+  int32_t* tile_items = nullptr; // [ntiles+1] CSR into items
+  void* partial = nullptr;       // [nitems][tile_px] partial tiles
+  int64_t partial_elem_bytes = 0;
+  int64_t plan_bytes = 0, partial_bytes = 0;
+};
+
+namespace {
+
+constexpr int64_t kSegment = int64_t(1) << 30;
+
+__global__ void __launch_bounds__(256) t2m_keys_kernel(const int32_t* __restrict__ ipix, int64_t n, int32_t nx,
+                                                       int32_t ny, int32_t tx, int32_t ty, int32_t ntile_y,
+                                                       uint32_t* __restrict__ key, uint32_t* __restrict__ idx,
+                                                       int64_t* nbad) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int32_t p = ipix[i];
+    uint32_t k = 0;
+    if (p < 0 || (int64_t)p >= (int64_t)nx * ny) {
+      atomicAdd((unsigned long long*)nbad, 1ull);
+    } else {
+      int32_t x = p / ny, y = p - x * ny;
+      k = (uint32_t)((x / tx) * ntile_y + (y / ty));
+    }
+    key[i] = k;
+    idx[i] = (uint32_t)i;
+  }
+}
+
+__global__ void __launch_bounds__(256) t2m_count_kernel(const uint32_t* __restrict__ key, int64_t n,
+                                                        int32_t* __restrict__ count) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) atomicAdd(count + key[i], 1);
+}
+
+// Writes local pixel per sorted entry.
+__global__ void __launch_bounds__(256) t2m_lpix_kernel(const int32_t* __restrict__ ipix, int64_t seg_base,
+                                                       const uint32_t* __restrict__ idx, int64_t n, int32_t ny,
+                                                       int32_t tx, int32_t ty, uint16_t* __restrict__ lpix) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int32_t p = ipix[seg_base + idx[i]];
+    int32_t x = p / ny, y = p - x * ny;
+    lpix[i] = (uint16_t)((x % tx) * ty + (y % ty));
+  }
+}
+
+// One wave per 64-entry group: order the group's entries by local pixel
+// (ties by position, so the order is unique) with a bitonic sort in
+// registers.  Groups never straddle items.
+__global__ void __launch_bounds__(256) t2m_groupsort_kernel(const T2MItem* __restrict__ items, int64_t nitems,
+                                                            const int64_t* __restrict__ group_base,
+                                                            int64_t ngroups, uint32_t* __restrict__ entry,
+                                                            uint16_t* __restrict__ lpix) {
+  int lane = threadIdx.x & 63;
+  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (g >= ngroups) return;  // whole wave exits together
+  // locate item by binary search on group_base
+  int64_t lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (group_base[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const T2MItem it = items[lo];
+  int64_t gi = g - group_base[lo];
+  int64_t pos0 = it.begin + gi * 64;
+  int64_t end = it.begin + it.count;
+  int64_t pos = pos0 + lane;
+  bool valid = pos < end;
+  uint32_t lp = valid ? lpix[pos] : 0xFFFFu;
+  uint32_t e = valid ? entry[pos] : 0u;
+  uint32_t key = (lp << 6) | (uint32_t)lane;
+  for (int k = 2; k <= 64; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint32_t other = __shfl_xor(key, j, 64);
+      bool up = ((lane & k) == 0);
+      bool lower = ((lane & j) == 0);
+      uint32_t mn = min(key, other), mx = max(key, other);
+      key = (lower == up) ? mn : mx;
+    }
+  }
+  int src = (int)(key & 63u);
+  uint32_t e_sorted = __shfl(e, src, 64);
+  if (valid) {
+    entry[pos] = e_sorted;
+    lpix[pos] = (uint16_t)(key >> 6);
+  }
+}
+
+// Exec stage 1: one workgroup (4 waves) per item.
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
+                                                        const uint32_t* __restrict__ entry,
+                                                        const uint16_t* __restrict__ lpix,
+                                                        const T* __restrict__ dat, int tile_px,
+                                                        P* __restrict__ partial) {
+  extern __shared__ unsigned char smem_raw[];
+  T* tiles = reinterpret_cast<T*>(smem_raw);  // [4][tile_px]
+  const T2MItem it = items[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int p = threadIdx.x; p < 4 * tile_px; p += 256) tiles[p] = T(0);
+  __syncthreads();
+  T* mine = tiles + wave * tile_px;
+  const T* base = dat + it.seg_base;
+  const int64_t end = it.begin + it.count;
+  const int ngroups = (it.count + 63) >> 6;
+  for (int g = wave; g < ngroups; g += 4) {
+    int64_t pos = it.begin + (int64_t)g * 64 + lane;
+    bool valid = pos < end;
+    int lp = valid ? (int)lpix[pos] : -1 - lane;  // distinct sentinels never match
+    T v = valid ? base[entry[pos]] : T(0);
+    // inclusive segmented scan over runs of equal lp (runs are contiguous)
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      T u = __shfl_up(v, off, 64);
+      int q = __shfl_up(lp, off, 64);
+      if (lane >= off && q == lp) v += u;
+    }
+    int nxt = __shfl_down(lp, 1, 64);
+    bool tail = valid && (lane == 63 || nxt != lp);
+    if (tail) mine[lp] += v;
+  }
+  __syncthreads();
+  P* out = partial + (int64_t)blockIdx.x * tile_px;
+  for (int p = threadIdx.x; p < tile_px; p += 256) {
+    T s = tiles[p];
+    s += tiles[tile_px + p];
+    s += tiles[2 * tile_px + p];
+    s += tiles[3 * tile_px + p];
+    out[p] = (P)s;
+  }
+}
+
+// Exec stage 2: one workgroup per map tile (all tiles, touched or not).
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restrict__ tile_items,
+                                                        const P* __restrict__ partial, int tile_x, int tile_y,
+                                                        int ntile_y, int nx, int ny, T* __restrict__ map,
+                                                        int accumulate) {
+  const int tile = blockIdx.x;
+  const int tx0 = (tile / ntile_y) * tile_x, ty0 = (tile % ntile_y) * tile_y;
+  const int i0 = tile_items[tile], i1 = tile_items[tile + 1];
+  const int tile_px = tile_x * tile_y;
+  for (int p = threadIdx.x; p < tile_px; p += blockDim.x) {
+    int x = tx0 + p / tile_y, y = ty0 + p % tile_y;
+    if (x >= nx || y >= ny) continue;
+    T s = accumulate ? map[(int64_t)x * ny + y] : T(0);
+    T acc = T(0);
+    bool any = false;
+    for (int i = i0; i < i1; i++) {
+      T v = (T)partial[(int64_t)i * tile_px + p];
+      acc = any ? acc + v : v;
+      any = true;
+    }
+    map[(int64_t)x * ny + y] = any ? s + acc : s;
+  }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+      p = nullptr;
+      mk::set_error("hipMalloc(%zu) failed", bytes);
+      return MK_ERR_ALLOC;
+    }
+    return MK_OK;
+  }
+};
+
+void free_plan(mk_t2m_plan* pl) {
+  if (!pl) return;
+  if (pl->entry) (void)hipFree(pl->entry);
+  if (pl->lpix) (void)hipFree(pl->lpix);
+  if (pl->items) (void)hipFree(pl->items);
+  if (pl->tile_items) (void)hipFree(pl->tile_items);
+  if (pl->partial) (void)hipFree(pl->partial);
+  delete pl;
+}
+
+}  // namespace
+
+extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx, int32_t ny, int32_t tile_x,
+                                  int32_t tile_y, int32_t item_samples, mk_t2m_plan** out, void* stream_v) {
+  MK_REQUIRE(out != nullptr, MK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  MK_REQUIRE(nsamp >= 0 && (nsamp == 0 || ipix), MK_ERR_ARG, "bad ipix/nsamp");
+  MK_REQUIRE(nx > 0 && ny > 0 && (int64_t)nx * ny <= INT32_MAX, MK_ERR_ARG, "bad map size");
+  MK_REQUIRE(tile_x > 0 && tile_y > 0 && tile_x * tile_y <= 8192, MK_ERR_ARG,
+             "tile must have 1..8192 pixels (got %d x %d)", tile_x, tile_y);
+  MK_REQUIRE(item_samples >= 64 && item_samples % 64 == 0, MK_ERR_ARG, "item_samples must be a positive multiple of 64");
+  hipStream_t stream = mk::as_stream(stream_v);
+  mk_t2m_plan* pl = new mk_t2m_plan();
+  pl->nsamp = nsamp; pl->nx = nx; pl->ny = ny; pl->tile_x = tile_x; pl->tile_y = tile_y;
+  pl->ntile_x = (int32_t)mk::cdiv(nx, tile_x);
+  pl->ntile_y = (int32_t)mk::cdiv(ny, tile_y);
+  pl->item_samples = item_samples;
+  const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
+  const int tile_px = tile_x * tile_y;
+  auto fail = [&](int code) { free_plan(pl); return code; };
+
+  if (hipMalloc((void**)&pl->entry, std::max<int64_t>(nsamp, 1) * 4) != hipSuccess ||
+      hipMalloc((void**)&pl->lpix, std::max<int64_t>(nsamp, 1) * 2) != hipSuccess) {
+    mk::set_error("t2m plan: hipMalloc of %lld entries failed", (long long)nsamp);
+    return fail(MK_ERR_ALLOC);
+  }
+  std::vector<T2MItem> h_items;
+  const int64_t segmax = std::min<int64_t>(kSegment, std::max<int64_t>(nsamp, 1));
+  DevBuf keys, keys2, idx, cnt, nbad, tmp;
+  if (keys.alloc(segmax * 4) || keys2.alloc(segmax * 4) || idx.alloc(segmax * 4) || cnt.alloc(ntiles * 4 + 4) ||
+      nbad.alloc(8))
+    return fail(MK_ERR_ALLOC);
+  size_t tmp_bytes = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t*)keys.p, (uint32_t*)keys2.p,
+                                         (uint32_t*)idx.p, (uint32_t*)pl->entry, (int)segmax, 0, 32,
+                                         stream) != hipSuccess)
+    return fail((mk::set_error("hipcub sort sizing failed"), MK_ERR_HIP));
+  if (tmp.alloc(tmp_bytes)) return fail(MK_ERR_ALLOC);
+  int end_bit = 1;
+  while ((int64_t(1) << end_bit) < ntiles) end_bit++;
+  std::vector<int32_t> h_cnt(ntiles);
+  for (int64_t s0 = 0; s0 < nsamp; s0 += kSegment) {
+    const int64_t n = std::min<int64_t>(kSegment, nsamp - s0);
+    if (hipMemsetAsync(nbad.p, 0, 8, stream) != hipSuccess || hipMemsetAsync(cnt.p, 0, ntiles * 4, stream) != hipSuccess)
+      return fail((mk::set_error("memset failed"), MK_ERR_HIP));
+    t2m_keys_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix + s0, n, nx, ny, tile_x, tile_y, pl->ntile_y,
+                                                               (uint32_t*)keys.p, (uint32_t*)idx.p, (int64_t*)nbad.p);
+    if (mk::check_launch("t2m_keys_kernel")) return fail(MK_ERR_HIP);
+    int64_t h_bad = 0;
+    if (hipMemcpyAsync(&h_bad, nbad.p, 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return fail((mk::set_error("t2m plan: sync failed"), MK_ERR_HIP));
+    if (h_bad) {
+      mk::set_error("t2m plan: %lld samples have ipix outside [0, %d*%d)", (long long)h_bad, nx, ny);
+      return fail(MK_ERR_RANGE);
+    }
+    size_t tb = tmp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, (uint32_t*)keys.p, (uint32_t*)keys2.p, (uint32_t*)idx.p,
+                                           (uint32_t*)(pl->entry + s0), (int)n, 0, end_bit, stream) != hipSuccess)
+      return fail((mk::set_error("hipcub sort failed"), MK_ERR_HIP));
+    t2m_count_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>((uint32_t*)keys2.p, n, (int32_t*)cnt.p);
+    t2m_lpix_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix, s0, pl->entry + s0, n, ny, tile_x, tile_y,
+                                                               pl->lpix + s0);
+    if (mk::check_launch("t2m_lpix_kernel")) return fail(MK_ERR_HIP);
+    if (hipMemcpyAsync(h_cnt.data(), cnt.p, ntiles * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return fail((mk::set_error("t2m plan: count copy failed"), MK_ERR_HIP));
+    int64_t pos = s0;
+    for (int64_t t = 0; t < ntiles; t++) {
+      int64_t c = h_cnt[t];
+      for (int64_t o = 0; o < c; o += item_samples) {
+        T2MItem it;
+        it.seg_base = s0;
+        it.begin = pos + o;
+        it.count = (int32_t)std::min<int64_t>(item_samples, c - o);
+        it.tile = (int32_t)t;
+        h_items.push_back(it);
+      }
+      pos += c;
+    }
+  }
+  // items grouped by tile (stable: segment order, then position)
+  std::stable_sort(h_items.begin(), h_items.end(), [](const T2MItem& a, const T2MItem& b) { return a.tile < b.tile; });
+  pl->nitems = (int64_t)h_items.size();
+  std::vector<int32_t> h_tile_items(ntiles + 1, 0);
+  std::vector<int64_t> h_group_base(std::max<int64_t>(pl->nitems, 1), 0);
+  int64_t ngroups = 0;
+  for (int64_t i = 0; i < pl->nitems; i++) {
+    h_tile_items[h_items[i].tile + 1]++;
+    h_group_base[i] = ngroups;
+    ngroups += (h_items[i].count + 63) / 64;
+    pl->max_item = std::max<int64_t>(pl->max_item, h_items[i].count);
+  }
+  for (int64_t t = 0; t < ntiles; t++) {
+    if (h_tile_items[t + 1]) pl->ntiles_touched++;
+    h_tile_items[t + 1] += h_tile_items[t];
+  }
+  MK_REQUIRE(pl->nitems < INT32_MAX, fail(MK_ERR_ARG), "too many items");
+  if (hipMalloc((void**)&pl->items, std::max<int64_t>(pl->nitems, 1) * sizeof(T2MItem)) != hipSuccess ||
+      hipMalloc((void**)&pl->tile_items, (ntiles + 1) * 4) != hipSuccess)
+    return fail((mk::set_error("t2m plan: item alloc failed"), MK_ERR_ALLOC));
+  DevBuf gb;
+  if (gb.alloc(h_group_base.size() * 8)) return fail(MK_ERR_ALLOC);
+  if (hipMemcpyAsync(pl->items, h_items.data(), pl->nitems * sizeof(T2MItem), hipMemcpyHostToDevice, stream) ||
+      hipMemcpyAsync(pl->tile_items, h_tile_items.data(), (ntiles + 1) * 4, hipMemcpyHostToDevice, stream) ||
+      hipMemcpyAsync(gb.p, h_group_base.data(), h_group_base.size() * 8, hipMemcpyHostToDevice, stream))
+    return fail((mk::set_error("t2m plan: upload failed"), MK_ERR_HIP));
+  if (ngroups > 0) {
+    t2m_groupsort_kernel<<<(unsigned)mk::cdiv(ngroups * 64, 256), 256, 0, stream>>>(
+        pl->items, pl->nitems, (const int64_t*)gb.p, ngroups, pl->entry, pl->lpix);
+    if (mk::check_launch("t2m_groupsort_kernel")) return fail(MK_ERR_HIP);
+  }
+  // partial tiles: 8-byte elements when affordable (FP64 exec), else 4
+  int64_t pb8 = std::max<int64_t>(pl->nitems, 1) * tile_px * 8;
+  pl->partial_elem_bytes = (pb8 <= (int64_t(1) << 30)) ? 8 : 4;
+  pl->partial_bytes = std::max<int64_t>(pl->nitems, 1) * tile_px * pl->partial_elem_bytes;
+  if (hipMalloc(&pl->partial, pl->partial_bytes) != hipSuccess)
+    return fail((mk::set_error("t2m plan: partial alloc (%lld B) failed", (long long)pl->partial_bytes), MK_ERR_ALLOC));
+  if (hipStreamSynchronize(stream) != hipSuccess) return fail((mk::set_error("t2m plan: sync failed"), MK_ERR_HIP));
+  pl->plan_bytes = nsamp * 6 + pl->nitems * (int64_t)sizeof(T2MItem) + (ntiles + 1) * 4;
+  *out = pl;
+  return MK_OK;
+}
+
+namespace {
+template <typename T>
+int t2m_exec(const mk_t2m_plan* pl, T* map, const T* dat, int accumulate, void* stream_v) {
+  MK_REQUIRE(pl && map, MK_ERR_ARG, "null plan/map");
+  MK_REQUIRE(pl->nsamp == 0 || dat, MK_ERR_ARG, "null dat");
+  hipStream_t stream = mk::as_stream(stream_v);
+  const int tile_px = pl->tile_x * pl->tile_y;
+  const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
+  const size_t lds = 4 * (size_t)tile_px * sizeof(T);
+  MK_REQUIRE(lds <= 65536, MK_ERR_ARG, "tile too large for LDS with this precision");
+  if (pl->nitems > 0) {
+    if (sizeof(T) == 8 && pl->partial_elem_bytes == 8) {
+      t2m_items_kernel<T, double><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix, dat,
+                                                                              tile_px, (double*)pl->partial);
+    } else {
+      MK_REQUIRE(sizeof(T) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
+      t2m_items_kernel<T, float><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix, dat,
+                                                                             tile_px, (float*)pl->partial);
+    }
+    int st = mk::check_launch("t2m_items_kernel");
+    if (st) return st;
+  }
+  if (sizeof(T) == 8 && pl->partial_elem_bytes == 8)
+    t2m_tiles_kernel<T, double><<<(unsigned)ntiles, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial,
+                                                                      pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx,
+                                                                      pl->ny, map, accumulate);
+  else
+    t2m_tiles_kernel<T, float><<<(unsigned)ntiles, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial,
+                                                                     pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx,
+                                                                     pl->ny, map, accumulate);
+  return mk::check_launch("t2m_tiles_kernel");
+}
+}  // namespace
+
+extern "C" int mk_t2m_plan_exec_f32(const mk_t2m_plan* plan, float* map, const float* dat, int accumulate,
+                                    void* stream) {
+  return t2m_exec<float>(plan, map, dat, accumulate, stream);
+}
+extern "C" int mk_t2m_plan_exec_f64(const mk_t2m_plan* plan, double* map, const double* dat, int accumulate,
+                                    void* stream) {
+  return t2m_exec<double>(plan, map, dat, accumulate, stream);
+}
+extern "C" int mk_t2m_plan_info(const mk_t2m_plan* pl, int64_t* info) {
+  MK_REQUIRE(pl && info, MK_ERR_ARG, "null pointer");
+  info[0] = pl->nsamp;
+  info[1] = pl->nitems;
+  info[2] = pl->ntiles_touched;
+  info[3] = pl->tile_x;
+  info[4] = pl->tile_y;
+  info[5] = pl->plan_bytes;
+  info[6] = pl->partial_bytes;
+  info[7] = pl->max_item;
+  return MK_OK;
+}
+extern "C" int mk_t2m_plan_destroy(mk_t2m_plan* pl) {
+  free_plan(pl);
+  return MK_OK;
+}

@@ -1,0 +1,103 @@
+"""ctypes bindings for libmkgpu (declared in include/mkgpu.h).
+
+Counterpart of the reference's argtypes tables (minkasi/lib/minkasi.py:10-225,
+minkasi/lib/mkfftw.py:7-95).  Every entry point returns a status; ``check``
+turns a non-zero status into ``MkError`` carrying ``mk_last_error()``.  The
+reference's C functions return void and never fail; raising here is a
+deliberate divergence.
+"""
+import ctypes
+
+from ._mkgpu_init import libmkgpu as lib, libmkgpu_path
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+D = ctypes.c_double
+
+MK_OK = 0
+MK_ERR_ARG = 1
+MK_ERR_HIP = 2
+MK_ERR_RANGE = 3
+MK_ERR_NUMERIC = 4
+MK_ERR_ALLOC = 5
+
+# device scalar block layout (MK_SC_* in mkgpu.h)
+SC_ZR, SC_PAP, SC_ALPHA, SC_BETA, SC_RR, SC_FLAG, SC_DOT = 0, 1, 2, 3, 4, 5, 6
+SC_COUNT = 8
+
+
+class MkError(RuntimeError):
+    def __init__(self, name, status, msg):
+        super().__init__("%s failed (status %d): %s" % (name, status, msg))
+        self.status = status
+
+
+_SIGS = {
+    "mk_last_error": ([], ctypes.c_char_p),
+    "mk_version": ([], I),
+    "mk_reduce_workspace_bytes": ([], I64),
+    "mk_pix_from_radec_car": ([P, P, I64, P, I32, I32, P, P, P], I),
+    "mk_world2pix_car": ([P, P, I64, P, P, P, P], I),
+    "mk_pix_from_raster_car": ([P, P, P, P, I64, I64, P, I32, I32, P, P, P], I),
+    "mk_map2tod_f32": ([P, P, I64, P, I, P], I),
+    "mk_map2tod_f64": ([P, P, I64, P, I, P], I),
+    "mk_tod2map_atomic_f32": ([P, P, I64, P, P], I),
+    "mk_tod2map_atomic_f64": ([P, P, I64, P, P], I),
+    "mk_t2m_plan_create": ([P, I64, I32, I32, I32, I32, I32, ctypes.POINTER(P), P], I),
+    "mk_t2m_plan_exec_f32": ([P, P, P, I, P], I),
+    "mk_t2m_plan_exec_f64": ([P, P, P, I, P], I),
+    "mk_t2m_plan_info": ([P, P], I),
+    "mk_t2m_plan_destroy": ([P], I),
+    "mk_dot_f32": ([P, P, I64, P, P, P], I),
+    "mk_dot_f64": ([P, P, I64, P, P, P], I),
+    "mk_axpy_f32": ([P, P, D, I64, P], I),
+    "mk_mul_f32": ([P, P, P, I64, P], I),
+    "mk_pcg_alpha": ([P, P, I64, P, P, P], I),
+    "mk_pcg_xr": ([P, P, P, P, I64, P, P, P], I),
+    "mk_pcg_xrz_jacobi": ([P, P, P, P, P, P, I64, P, P, P], I),
+    "mk_pcg_beta": ([P, P, I64, P, P, P], I),
+    "mk_pcg_p": ([P, P, I64, P, P], I),
+    "mk_pcg_init": ([P, P, P, I64, P, P, P], I),
+    "mk_dct1_workspace_bytes": ([I64, I64, I], I64),
+    "mk_dct1_f64": ([P, P, I64, I64, P, I64, P], I),
+    "mk_dct1_f32": ([P, P, I64, I64, P, I64, P], I),
+    "mk_filt_plan_create": ([I, P, I, ctypes.POINTER(P)], I),
+    "mk_filt_plan_set_weights": ([P, P, P], I),
+    "mk_filt_plan_set_halve": ([P, I], I),
+    "mk_filt_plan_exec_f32": ([P, P, P, P], I),
+    "mk_filt_plan_exec_f64": ([P, P, P, P], I),
+    "mk_filt_plan_info": ([P, P], I),
+    "mk_filt_plan_geometry": ([P, P], I),
+    "mk_filt_plan_destroy": ([P], I),
+}
+
+_fns = {}
+for _name, (_args, _res) in _SIGS.items():
+    if hasattr(lib, _name):
+        _f = getattr(lib, _name)
+        _f.argtypes = _args
+        _f.restype = _res
+        _fns[_name] = _f
+
+
+def fn(name):
+    try:
+        return _fns[name]
+    except KeyError:
+        raise MkError(name, MK_ERR_ARG, "symbol not exported by %s" % libmkgpu_path)
+
+
+def check(name, *args):
+    """Calls ``name`` and raises MkError on a non-zero status."""
+    st = fn(name)(*args)
+    if st != MK_OK:
+        msg = lib.mk_last_error()
+        raise MkError(name, st, msg.decode() if msg else "")
+    return st
+
+
+def last_error():
+    m = lib.mk_last_error()
+    return m.decode() if m else ""

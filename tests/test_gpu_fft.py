@@ -1,0 +1,150 @@
+"""GPU DCT-I (Bluestein) and the batched C W C noise filter against scipy's
+DCT-I, the reference's own no-FFTW path (minkasi/tools/py_fft.py:239)."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+from tests.helpers import max_rel_err, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    from minkasi_amd import device
+    from minkasi_amd.lib import mkgpu
+    return torch, device, mkgpu
+
+
+def gpu_dct1(env, x, dtype):
+    torch, device, mkgpu = env
+    nrow, n = x.shape
+    eb = 8 if dtype == np.float64 else 4
+    nb = int(mkgpu.lib.mk_dct1_workspace_bytes(nrow, n, eb))
+    work = torch.empty(nb // 8 + 64, dtype=torch.float64, device=device.device())
+    xt = device.to_device(x.astype(dtype))
+    yt = torch.empty_like(xt)
+    name = "mk_dct1_f64" if eb == 8 else "mk_dct1_f32"
+    mkgpu.check(name, xt.data_ptr(), yt.data_ptr(), nrow, n, work.data_ptr(), work.numel() * 8, device.stream_ptr())
+    return yt.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 17, 100, 1001, 4097, 65536, 40001, 262144])
+def test_dct1_f64_matches_scipy(env, n):
+    rng = np.random.default_rng(n)
+    nrow = 3 if n < 100000 else 1
+    x = rng.standard_normal((nrow, n))
+    ref = sf.dct(x, type=1, axis=-1)
+    y = gpu_dct1(env, x, np.float64)
+    assert rel_err(y, ref) < 1e-13, n
+    assert max_rel_err(y, ref) < 1e-12, n
+
+
+@pytest.mark.parametrize("n", [5, 1000, 65536, 100001])
+def test_dct1_f32_matches_scipy(env, n):
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal((2, n))
+    ref = sf.dct(x, type=1, axis=-1)
+    y = gpu_dct1(env, x, np.float32)
+    assert rel_err(y, ref) < 2e-6, n
+
+
+def test_dct1_involution_large(env):
+    # DCT-I twice = 2(n-1) I, at a config-3 row length
+    n = 262144
+    x = np.random.default_rng(9).standard_normal((1, n))
+    y = gpu_dct1(env, gpu_dct1(env, x, np.float64), np.float64)
+    assert rel_err(y / (2 * (n - 1)), x) < 1e-12
+
+
+def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False):
+    """Batched filter over segments of (rows, n); returns gpu, ref."""
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(seed)
+    desc = []
+    off = 0
+    xs, ws = [], []
+    for rows, n in rows_n:
+        desc += [rows, n, off, n, off, n]
+        off += rows * n
+        xs.append(rng.standard_normal((rows, n)))
+        ws.append(np.abs(rng.standard_normal((rows, n))) + 0.05)
+    desc = np.asarray(desc, dtype=np.int64)
+    eb = 8 if dtype == np.float64 else 4
+    h = ctypes.c_void_p()
+    mkgpu.check("mk_filt_plan_create", len(rows_n), desc.ctypes.data, eb, ctypes.byref(h))
+    try:
+        if wdct_from_gpu:
+            wd = np.concatenate([gpu_dct1(env, w, np.float64).ravel() for w in ws])
+        else:
+            wd = np.concatenate([sf.dct(w, type=1, axis=-1).ravel() for w in ws])
+        wdt = device.to_device(wd)
+        mkgpu.check("mk_filt_plan_set_weights", h, wdt.data_ptr(), device.stream_ptr())
+        mkgpu.check("mk_filt_plan_set_halve", h, int(halve))
+        xt = device.to_device(np.concatenate([x.ravel() for x in xs]).astype(dtype))
+        yt = torch.empty_like(xt)
+        name = "mk_filt_plan_exec_f64" if eb == 8 else "mk_filt_plan_exec_f32"
+        mkgpu.check(name, h, xt.data_ptr(), yt.data_ptr(), device.stream_ptr())
+        out = yt.cpu().numpy()
+        info = np.zeros(8, dtype=np.int64)
+        mkgpu.check("mk_filt_plan_info", h, info.ctypes.data)
+    finally:
+        mkgpu.check("mk_filt_plan_destroy", h)
+    refs = []
+    for x, w in zip(xs, ws):
+        r = sf.dct(sf.dct(x, type=1, axis=-1) * w, type=1, axis=-1)
+        if halve:
+            r[:, 0] *= 0.5
+            r[:, -1] *= 0.5
+        refs.append(r.ravel())
+    return out, np.concatenate(refs), info
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_filter_batched_matches_dct_pair(env, dtype):
+    segs = [(3, 17), (2, 2), (4, 1001), (2, 65536), (3, 40001), (1, 131073)]
+    out, ref, info = _filter(env, segs, dtype, halve=True)
+    tol = 1e-12 if dtype == np.float64 else 5e-6
+    assert rel_err(out, ref) < tol
+    off = 0
+    for rows, n in segs:
+        o = out[off:off + rows * n]
+        r = ref[off:off + rows * n]
+        assert rel_err(o, r) < tol, (rows, n)
+        off += rows * n
+
+
+def test_filter_weights_from_gpu_dct(env):
+    out, ref, _ = _filter(env, [(2, 5000), (2, 77777)], np.float64, halve=False, seed=3, wdct_from_gpu=True)
+    assert rel_err(out, ref) < 1e-12
+
+
+def test_filter_is_symmetric_and_kills_constants(env):
+    """<a, F b> = <F a, b> and F(const) = 0 when W[0] = 0 (noise.py:677)."""
+    torch, device, mkgpu = env
+    n = 30001
+    rng = np.random.default_rng(4)
+    w = np.abs(rng.standard_normal((1, n))) + 0.1
+    w[:, 0] = 0
+    desc = np.asarray([2, n, 0, n, 0, n], dtype=np.int64)
+    h = ctypes.c_void_p()
+    mkgpu.check("mk_filt_plan_create", 1, desc.ctypes.data, 8, ctypes.byref(h))
+    try:
+        wd = np.concatenate([sf.dct(w, type=1, axis=-1)] * 2).ravel()
+        wdt = device.to_device(wd)
+        mkgpu.check("mk_filt_plan_set_weights", h, wdt.data_ptr(), device.stream_ptr())
+        mkgpu.check("mk_filt_plan_set_halve", h, 1)
+        a, b = rng.standard_normal(n), rng.standard_normal(n)
+        xt = device.to_device(np.stack([a, b]))
+        yt = torch.empty_like(xt)
+        mkgpu.check("mk_filt_plan_exec_f64", h, xt.data_ptr(), yt.data_ptr(), device.stream_ptr())
+        Fa, Fb = yt.cpu().numpy()
+        assert abs(np.dot(a, Fb) - np.dot(Fa, b)) < 1e-10 * np.linalg.norm(Fa) * np.linalg.norm(b)
+        ct = device.to_device(np.ones((2, n)) * 3.0)
+        mkgpu.check("mk_filt_plan_exec_f64", h, ct.data_ptr(), yt.data_ptr(), device.stream_ptr())
+        assert np.abs(yt.cpu().numpy()).max() < 1e-9 * 3.0 * n
+    finally:
+        mkgpu.check("mk_filt_plan_destroy", h)
