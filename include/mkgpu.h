@@ -36,9 +36,7 @@ const char *mk_last_error(void);
 /* Library ABI version (increments when a signature changes). */
 int mk_version(void);
 /* Number of bytes of workspace the reductions below need (device memory,
- * 256-byte aligned, contents may be garbage on entry; the library leaves
- * its counters at zero on exit so one workspace can be reused without
- * clearing). */
+ * 256-byte aligned; contents may be garbage on entry, no clearing needed). */
 int64_t mk_reduce_workspace_bytes(void);
 
 /* ------------------------------------------------------------------ */
@@ -192,6 +190,11 @@ int mk_filt_plan_create(int nseg, const int64_t *desc, int elem_bytes,
  * exactly like the exec input (in_off/in_stride of desc). */
 int mk_filt_plan_set_weights(mk_filt_plan *plan, const double *wdct,
                              void *stream);
+/* Segment `seg` of `plan` uses the spectrum of segment `src_seg` of `src`
+ * (same rows, n and precision) instead of its own; `src` must outlive
+ * `plan`.  Lets one batched plan reuse the per-TOD noise models' spectra. */
+int mk_filt_plan_borrow_spectrum(mk_filt_plan *plan, int seg,
+                                 const mk_filt_plan *src, int src_seg);
 /* halve_edges=1 multiplies output samples 0 and n-1 by 1/2, which is the
  * reference's dd[:,0] *= .5; dd[:,-1] *= .5 (noise.py:710-711) moved before
  * the back-rotation it commutes with. */
@@ -206,6 +209,50 @@ int mk_filt_plan_info(const mk_filt_plan *plan, int64_t *info);
 /* geom[4*i..4*i+3] = P1, P2, columns per item, P of segment i. */
 int mk_filt_plan_geometry(const mk_filt_plan *plan, int64_t *geom);
 int mk_filt_plan_destroy(mk_filt_plan *plan);
+
+/* ------------------------------------------------------------------ */
+/* Detector rotation of apply_noise, dat_rot = v.dat and dd = vT.dat_rot
+ * (minkasi/mapmaking/noise.py:704, :709; numpy BLAS in the reference), as a
+ * split-BF16 MFMA GEMM with FP32 accumulation, batched over TODs.
+ * A (v or vT) is stored as hi/mid/lo bf16 planes (hi = bf16(a),
+ * mid = bf16(a-hi), lo = bf16(a-hi-mid)), row-major, padded to
+ * M_pad = ceil(M/128)*128 rows and lda = ceil(K/32)*32 columns (zeros).
+ * mk_split_bf16_padded_dims returns M_pad*lda and sets *M_pad, *lda. */
+int64_t mk_split_bf16_padded_dims(int64_t M, int64_t K, int64_t *M_pad,
+                                  int64_t *lda);
+int mk_split_bf16_matrix(const double *a, int64_t M, int64_t K,
+                         int64_t lda_src, void *hi, void *mid, void *lo,
+                         void *stream);
+/* Batched C = A.B.  desc: 11 int64 per segment: M, K, N, A_hi, A_mid,
+ * A_lo (device addresses), lda, b_off, b_ld, c_off, c_ld (element offsets
+ * into the exec-time FP32 base pointers).  mode 0: three products
+ * (hh + hm + mh, ~2^-18 relative per term); mode 1: six products (adds
+ * hl + lh + mm, ~FP32 accuracy). */
+typedef struct mk_rot_plan mk_rot_plan;
+int mk_rot_plan_create(int nseg, const int64_t *desc, int mode,
+                       mk_rot_plan **plan);
+int mk_rot_plan_exec(const mk_rot_plan *plan, const float *b_base,
+                     float *c_base, void *stream);
+int mk_rot_plan_destroy(mk_rot_plan *plan);
+/* FP64 setup GEMMs (NoiseSmoothedSVD.__init__, noise.py:650 and :667):
+ * C = A.B (A FP64; B FP32 if b_is_f32 else FP64), and the Gram matrix
+ * G = D.D^T (M x M) that replaces the SVD's left singular vectors
+ * (eigenvectors of D.D^T).  work: mk_gram_workspace_bytes(M, n) bytes. */
+int mk_gemm_nn_f64(int64_t M, int64_t N, int64_t K, const double *A,
+                   int64_t lda, const void *B, int b_is_f32, int64_t ldb,
+                   double *C, int64_t ldc, void *stream);
+int64_t mk_gram_workspace_bytes(int64_t M, int64_t n);
+int mk_gram_f64(int64_t M, int64_t n, const void *D, int d_is_f32,
+                int64_t ldd, double *G, void *work, int64_t work_bytes,
+                void *stream);
+/* Spectrum smoothing (minkasi/tools/smooth.py:49-78 smooth_many_vecs and
+ * noise.py:674-678): out_k = sum_{t=t_lo..t_hi} g[|t|] P(k-t) over the even
+ * periodic extension of each row (period 2(n-1)); P = x^2 if flags&1 else
+ * x; flags&2 inverts the result and zeroes bin 0.  out32 (may be NULL)
+ * receives an FP32 copy.  g: device FP64, max(|t_lo|, t_hi)+1 taps. */
+int mk_smooth_rows_f64(const double *x, double *out, float *out32,
+                       int64_t nrow, int64_t n, const double *g, int32_t t_lo,
+                       int32_t t_hi, int32_t flags, void *stream);
 
 #ifdef __cplusplus
 }

@@ -170,6 +170,7 @@ struct Seg {
   int64_t out_off, out_stride;  // output rows
   int64_t t_off, t_stride;      // T buffer rows (complex)
   int64_t s_off, s_stride;      // spectrum rows (filter: P+1 reals; Bluestein: P complex, stride 0)
+  void* s_ptr;                  // device address of this segment's spectrum rows (set at upload)
   int32_t rows, n, l1, l2, cw, flags;
   int64_t base1, base2, base3;  // first global item of K1, K2, K3
   int32_t per1, per2;           // items per row for K1/K3, K2
@@ -284,18 +285,18 @@ __global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ se
   fill_twiddles<T>(tw, l2);
   lds_fft<T, false>(X, tw, l2, nrow, P2, 1);
   if (MODE == kBSpec) {
-    cx<T>* srow = spec_c + sg.s_off;
+    cx<T>* srow = reinterpret_cast<cx<T>*>(sg.s_ptr);
     for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) srow[(int64_t)k1 * P2 + k2] = X[k2];
     return;
   }
   if (MODE == kBlue) {
-    const cx<T>* srow = spec_c + sg.s_off;
+    const cx<T>* srow = reinterpret_cast<const cx<T>*>(sg.s_ptr);
     for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) X[k2] = cmul(X[k2], srow[(int64_t)k1 * P2 + k2]);
     __syncthreads();
   } else {
     // unpack the real spectrum A_k = E_k + w_L^k O_k for k = kk1 + P1*k2
     const int64_t L = 2 * P;
-    T* srow = spec_r + sg.s_off + (int64_t)row * sg.s_stride;
+    T* srow = reinterpret_cast<T*>(sg.s_ptr) + (int64_t)row * sg.s_stride;
     T YPval = T(0);
     for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
       int r = idx >> l2, k2 = idx & (P2 - 1);
@@ -486,9 +487,18 @@ struct mk_filt_plan {
   void* spec = nullptr;    // real, P+1 per row
   size_t lds1 = 0, lds2 = 0;
   bool have_spec = false;
+  std::vector<char> borrowed;
 };
 
 namespace {
+int upload(mk_filt_plan* p) {
+  if (p->nseg && hipMemcpy(p->d_segs, p->h_segs.data(), p->nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
+    mk::set_error("filter plan: descriptor upload failed");
+    return MK_ERR_HIP;
+  }
+  return MK_OK;
+}
+
 void free_filt(mk_filt_plan* p) {
   if (!p) return;
   if (p->d_segs) (void)hipFree(p->d_segs);
@@ -562,6 +572,7 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
   allow_lds_all();
   mk_filt_plan* p = new mk_filt_plan();
   p->nseg = nseg;
+  p->borrowed.assign(std::max(1, nseg), 0);
   p->elem_bytes = elem_bytes;
   int st = build_segs(nseg, desc, elem_bytes, true, p->h_segs, &p->n1, &p->n2, &p->t_elems, &p->s_elems, &p->lds1,
                       &p->lds2);
@@ -574,13 +585,30 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
     mk::set_error("filter plan: device allocation failed");
     return MK_ERR_ALLOC;
   }
-  if (nseg && hipMemcpy(p->d_segs, p->h_segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
+  for (auto& sg : p->h_segs) sg.s_ptr = (unsigned char*)p->spec + sg.s_off * elem_bytes;
+  if (upload(p)) {
     free_filt(p);
-    mk::set_error("filter plan: upload failed");
     return MK_ERR_HIP;
   }
   *out = p;
   return MK_OK;
+}
+
+extern "C" int mk_filt_plan_borrow_spectrum(mk_filt_plan* p, int seg, const mk_filt_plan* src, int src_seg) {
+  MK_REQUIRE(p && src, MK_ERR_ARG, "null plan");
+  MK_REQUIRE(seg >= 0 && seg < p->nseg && src_seg >= 0 && src_seg < src->nseg, MK_ERR_ARG, "segment out of range");
+  MK_REQUIRE(p->elem_bytes == src->elem_bytes, MK_ERR_ARG, "precision mismatch");
+  MK_REQUIRE(src->have_spec, MK_ERR_ARG, "source plan has no weights");
+  Seg& d = p->h_segs[seg];
+  const Seg& s = src->h_segs[src_seg];
+  MK_REQUIRE(d.rows == s.rows && d.n == s.n && d.l1 == s.l1 && d.l2 == s.l2, MK_ERR_ARG,
+             "segment geometry differs (rows %d/%d, n %d/%d)", d.rows, s.rows, d.n, s.n);
+  d.s_ptr = s.s_ptr;
+  p->borrowed[seg] = 1;
+  bool all = true;
+  for (int i = 0; i < p->nseg; i++) all = all && p->borrowed[i];
+  if (all) p->have_spec = true;
+  return upload(p) ? MK_ERR_HIP : MK_OK;
 }
 
 extern "C" int mk_filt_plan_destroy(mk_filt_plan* p) {
@@ -616,6 +644,8 @@ extern "C" int mk_filt_plan_geometry(const mk_filt_plan* p, int64_t* out) {
 
 extern "C" int mk_filt_plan_set_weights(mk_filt_plan* p, const double* wdct, void* stream) {
   MK_REQUIRE(p && (p->nseg == 0 || wdct), MK_ERR_ARG, "null argument");
+  for (int i = 0; i < p->nseg; i++)
+    MK_REQUIRE(!p->borrowed[i], MK_ERR_ARG, "plan borrows spectra; set_weights would overwrite the source plan");
   if (p->n1 == 0) { p->have_spec = true; return MK_OK; }
   hipStream_t st = mk::as_stream(stream);
   if (p->elem_bytes == 4) launch_fspec<float>(p, wdct, st);
@@ -646,11 +676,7 @@ extern "C" int mk_filt_plan_exec_f64(const mk_filt_plan* p, const double* in, do
 extern "C" int mk_filt_plan_set_halve(mk_filt_plan* p, int halve_edges) {
   MK_REQUIRE(p, MK_ERR_ARG, "null plan");
   for (auto& s : p->h_segs) s.flags = halve_edges ? 1 : 0;
-  if (p->nseg && hipMemcpy(p->d_segs, p->h_segs.data(), p->nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
-    mk::set_error("filter plan: upload failed");
-    return MK_ERR_HIP;
-  }
-  return MK_OK;
+  return upload(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -688,6 +714,7 @@ int dct1_impl(const TIO* x, TIO* y, int64_t nrow, int64_t n, void* work, int64_t
   cx<T>* tb = V + segs[0].t_stride;
   Seg s0 = segs[0];
   s0.s_off = 0;
+  s0.s_ptr = V;
   MK_HIP(hipMemcpyAsync(d_seg, &s0, sizeof(Seg), hipMemcpyHostToDevice, st));
   MK_HIP(hipStreamSynchronize(st));
   // spectrum of the conjugate chirp: a one-row pass (rows = 1) over the same geometry

@@ -1,0 +1,417 @@
+// Detector-space rotations of NoiseSmoothedSVD (minkasi/mapmaking/noise.py:
+// 650 svd, :667 v.d at setup, :704 v.d and :709 vT.d in apply_noise).
+//
+//  Hot path (apply_noise, every A.x): C = A.B with A = v or vT (ndet x ndet)
+//  and B a TOD (ndet x n, FP32), on MFMA as split BF16, accumulated in FP32
+//  by v_mfma_f32_32x32x16_bf16.  A and B are cut into bf16 pieces
+//  (h = RNE bf16 of x, m = bf16 of x-h, l = bf16 of x-h-m; for FP32 x the
+//  three pieces are exact).  Mode 0 keeps the three products of order <= 1,
+//  Ah.Bh + Ah.Bm + Am.Bh (about 2^-18 relative per product); mode 1 keeps
+//  the six of order <= 2 (adds Ah.Bl + Al.Bh + Am.Bm), about FP32 accuracy,
+//  which matters when a large common mode is rotated into modes the noise
+//  filter later amplifies.  Workgroup tile 128x128, four waves of 64x64
+//  (2x2 MFMA blocks), K staged through LDS 32 at a time; B is split while
+//  it is staged.  Every TOD of a batch is one
+//  segment of one launch; the block->tile map keeps the M-blocks that share
+//  a column block of B on one XCD so B is re-read from that XCD's L2.
+//
+//  Setup (once per TOD): the Gram matrix d.d^T and the rotation v.d in
+//  FP64 (plain tiled FMA kernels), feeding the host eigendecomposition that
+//  stands in for np.linalg.svd's left singular vectors.
+#include "common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 128, BN = 128, BK = 32, KPAD = 8;  // LDS row = 40 bf16 = 80 B
+
+struct RotSeg {
+  const __bf16* a_hi;  // M_pad x lda (zero padded: M_pad = ceil(M/128)*128, lda % 32 == 0)
+  const __bf16* a_mid;
+  const __bf16* a_lo;
+  int64_t lda;
+  int64_t b_off, b_ld;  // element offsets into the exec-time B base
+  int64_t c_off, c_ld;  // element offsets into the exec-time C base
+  int32_t M, K, N, mblocks, nblocks, pad_;
+  int64_t base;         // first global tile of this segment
+};
+
+__device__ inline int find_rseg(const RotSeg* segs, int nseg, int64_t g) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].base <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline void split3_bf16(float x, __bf16* h, __bf16* m, __bf16* l) {
+  __bf16 hh = (__bf16)x;  // round to nearest even
+  float r = x - (float)hh;  // exact
+  __bf16 mm = (__bf16)r;
+  *h = hh;
+  *m = mm;
+  *l = (__bf16)(r - (float)mm);
+}
+
+// NP = number of products: 3 (mode 0) or 6 (mode 1).
+template <int NP>
+__global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict__ segs, int nseg, int64_t ntiles,
+                                                       const float* __restrict__ Bbase, float* __restrict__ Cbase) {
+  constexpr int NPL = NP == 6 ? 3 : 2;  // bf16 planes per operand
+  __shared__ __attribute__((aligned(16))) __bf16 As[NPL][BM][BK + KPAD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NPL][BN][BK + KPAD];
+
+  // XCD-aware remap: hardware dispatches consecutive blocks round-robin over
+  // 8 XCDs; give each XCD a contiguous range of logical tiles.
+  const int64_t G = gridDim.x;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t logical = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (logical >= ntiles) return;
+  const int s = find_rseg(segs, nseg, logical);
+  const RotSeg sg = segs[s];
+  const int64_t local = logical - sg.base;
+  const int mb = (int)(local % sg.mblocks), nb = (int)(local / sg.mblocks);
+  const int m0 = mb * BM, n0 = nb * BN;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const float* B = Bbase + sg.b_off;
+  float* C = Cbase + sg.c_off;
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+  for (int k0 = 0; k0 < sg.K; k0 += BK) {
+    // A tile: 128 rows x 32 bf16 (padded storage: no bounds needed)
+    {
+      int row = tid >> 1, half = tid & 1;
+      const __bf16* planes[3] = {sg.a_hi, sg.a_mid, sg.a_lo};
+#pragma unroll
+      for (int pl = 0; pl < NPL; pl++) {
+        const uint4* src = reinterpret_cast<const uint4*>(planes[pl] + (int64_t)(m0 + row) * sg.lda + k0 + half * 16);
+        uint4 v0 = src[0], v1 = src[1];
+        uint4* dst = reinterpret_cast<uint4*>(&As[pl][row][half * 16]);
+        dst[0] = v0;
+        dst[1] = v1;
+      }
+    }
+    // B tile: 32 rows (k) x 128 columns (n), FP32 -> hi/lo, stored [n][k]
+#pragma unroll 4
+    for (int idx = tid; idx < BK * BN; idx += 256) {
+      int kk = idx >> 7, nn = idx & (BN - 1);
+      int gk = k0 + kk, gn = n0 + nn;
+      float x = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
+      __bf16 h, m, l;
+      split3_bf16(x, &h, &m, &l);
+      Bs[0][nn][kk] = h;
+      Bs[1][nn][kk] = m;
+      if (NPL == 3) Bs[NPL - 1][nn][kk] = l;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kq = ks + 8 * (lane >> 5);
+      bf16x8 a[NPL][2], b[NPL][2];
+#pragma unroll
+      for (int pl = 0; pl < NPL; pl++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          int r = wm * 64 + i * 32 + (lane & 31);
+          a[pl][i] = *reinterpret_cast<const bf16x8*>(&As[pl][r][kq]);
+          int c = wn * 64 + i * 32 + (lane & 31);
+          b[pl][i] = *reinterpret_cast<const bf16x8*>(&Bs[pl][c][kq]);
+        }
+      // smallest products first
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          if (NP == 6) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[NPL - 1][j], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // epilogue: D[row][col], row = 8*(r/4) + 4*(lane/32) + r%4, col = lane%32
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      int col = n0 + wn * 64 + j * 32 + (lane & 31);
+      if (col >= sg.N) continue;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row < sg.M) C[(int64_t)row * sg.c_ld + col] = acc[i][j][r];
+      }
+    }
+}
+
+// hi/mid/lo split of an FP64 matrix into padded row-major bf16 planes.
+__global__ void split_matrix_kernel(const double* __restrict__ a, int64_t M, int64_t K, int64_t lda_src,
+                                    __bf16* __restrict__ hi, __bf16* __restrict__ mid, __bf16* __restrict__ lo,
+                                    int64_t M_pad, int64_t lda) {
+  int64_t n = M_pad * lda;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = i / lda, c = i - r * lda;
+    double x = (r < M && c < K) ? a[r * lda_src + c] : 0.0;
+    __bf16 h = (__bf16)(float)x;
+    double r1 = x - (double)(float)h;
+    __bf16 m = (__bf16)(float)r1;
+    double r2 = r1 - (double)(float)m;
+    hi[i] = h;
+    mid[i] = m;
+    lo[i] = (__bf16)(float)r2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 setup GEMMs
+
+constexpr int TB = 64, TK = 16;
+
+// C[M x N] = A[M x K] . B[K x N]; A FP64 row-major (lda), B of type TB_ (ldb),
+// C FP64 (ldc).  64x64 tile per workgroup, 4x4 per thread.
+template <typename TBT>
+__global__ void __launch_bounds__(256) gemm_nn_f64(int M, int N, int K, const double* __restrict__ A, int64_t lda,
+                                                   const TBT* __restrict__ B, int64_t ldb, double* __restrict__ C,
+                                                   int64_t ldc) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  const int bm = blockIdx.y * TB, bn = blockIdx.x * TB;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    for (int idx = threadIdx.x; idx < TK * TB; idx += 256) {
+      int kk = idx / TB, mm = idx % TB;
+      int gm = bm + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : 0.0;
+      int gn = bn + mm;
+      Bs[kk][mm] = (gk < K && gn < N) ? (double)B[(int64_t)gk * ldb + gn] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int gm = bm + ty + 16 * i, gn = bn + tx + 16 * j;
+      if (gm < M && gn < N) C[(int64_t)gm * ldc + gn] = acc[i][j];
+    }
+}
+
+// Partial Gram matrices: P[chunk][M x M] = D[:, chunk] . D[:, chunk]^T.
+template <typename TD>
+__global__ void __launch_bounds__(256) gram_partial_f64(int M, int64_t n, const TD* __restrict__ D, int64_t ldd,
+                                                        int64_t chunk, double* __restrict__ P) {
+  __shared__ double As[TK][TB + 1];
+  __shared__ double Bs[TK][TB + 1];
+  const int bm = blockIdx.y * TB, bn = blockIdx.x * TB;
+  const int64_t c0 = (int64_t)blockIdx.z * chunk, c1 = min(n, c0 + chunk);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4] = {};
+  for (int64_t k0 = c0; k0 < c1; k0 += TK) {
+    for (int idx = threadIdx.x; idx < TK * TB; idx += 256) {
+      int mm = idx / TK, kk = idx % TK;  // consecutive threads walk along n
+      int64_t gk = k0 + kk;
+      int gm = bm + mm, gn = bn + mm;
+      As[kk][mm] = (gm < M && gk < c1) ? (double)D[(int64_t)gm * ldd + gk] : 0.0;
+      Bs[kk][mm] = (gn < M && gk < c1) ? (double)D[(int64_t)gn * ldd + gk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  double* out = P + (int64_t)blockIdx.z * M * M;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int gm = bm + ty + 16 * i, gn = bn + tx + 16 * j;
+      if (gm < M && gn < M) out[(int64_t)gm * M + gn] = acc[i][j];
+    }
+}
+
+__global__ void sum_partials_f64(const double* __restrict__ P, int64_t nchunk, int64_t mm, double* __restrict__ G) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mm; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t c = 0; c < nchunk; c++) s += P[c * mm + i];
+    G[i] = s;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct mk_rot_plan {
+  int nseg = 0;
+  int mode = 0;
+  int64_t ntiles = 0;
+  RotSeg* d_segs = nullptr;
+};
+
+extern "C" int64_t mk_split_bf16_padded_dims(int64_t M, int64_t K, int64_t* M_pad, int64_t* lda) {
+  if (M_pad) *M_pad = mk::cdiv(M, BM) * BM;
+  if (lda) *lda = mk::cdiv(K, BK) * BK;
+  return mk::cdiv(M, BM) * BM * mk::cdiv(K, BK) * BK;
+}
+
+extern "C" int mk_split_bf16_matrix(const double* a, int64_t M, int64_t K, int64_t lda_src, void* hi, void* mid,
+                                    void* lo, void* stream) {
+  MK_REQUIRE(a && hi && mid && lo && M > 0 && K > 0 && lda_src >= K, MK_ERR_ARG, "bad arguments");
+  int64_t M_pad, lda;
+  mk_split_bf16_padded_dims(M, K, &M_pad, &lda);
+  split_matrix_kernel<<<mk::grid_for(M_pad * lda, 256), 256, 0, mk::as_stream(stream)>>>(
+      a, M, K, lda_src, (__bf16*)hi, (__bf16*)mid, (__bf16*)lo, M_pad, lda);
+  return mk::check_launch("split_matrix_kernel");
+}
+
+// desc per segment (11 int64): M, K, N, a_hi, a_mid, a_lo (device addresses), lda, b_off, b_ld, c_off, c_ld
+extern "C" int mk_rot_plan_create(int nseg, const int64_t* desc, int mode, mk_rot_plan** out) {
+  MK_REQUIRE(out, MK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  MK_REQUIRE(nseg >= 0 && (nseg == 0 || desc), MK_ERR_ARG, "bad segment table");
+  MK_REQUIRE(mode == 0 || mode == 1, MK_ERR_ARG, "mode must be 0 (3 products) or 1 (6 products)");
+  std::vector<RotSeg> segs(nseg);
+  int64_t base = 0;
+  for (int i = 0; i < nseg; i++) {
+    const int64_t* d = desc + 11 * i;
+    RotSeg s{};
+    s.M = (int32_t)d[0]; s.K = (int32_t)d[1]; s.N = (int32_t)d[2];
+    s.a_hi = (const __bf16*)(uintptr_t)d[3];
+    s.a_mid = (const __bf16*)(uintptr_t)d[4];
+    s.a_lo = (const __bf16*)(uintptr_t)d[5];
+    s.lda = d[6];
+    s.b_off = d[7]; s.b_ld = d[8]; s.c_off = d[9]; s.c_ld = d[10];
+    MK_REQUIRE(d[0] > 0 && d[1] > 0 && d[2] >= 0 && d[0] < (1 << 20) && d[1] < (1 << 20) && d[2] < (1 << 30),
+               MK_ERR_ARG, "segment %d: bad sizes", i);
+    MK_REQUIRE(s.a_hi && s.a_mid && s.a_lo, MK_ERR_ARG, "segment %d: null A", i);
+    MK_REQUIRE(s.lda % BK == 0 && s.lda >= s.K, MK_ERR_ARG, "segment %d: lda must be a multiple of %d >= K", i, BK);
+    MK_REQUIRE(((uintptr_t)s.a_hi & 15) == 0 && ((uintptr_t)s.a_mid & 15) == 0 && ((uintptr_t)s.a_lo & 15) == 0,
+               MK_ERR_ARG, "A must be 16-B aligned");
+    s.mblocks = (int32_t)mk::cdiv(s.M, BM);
+    s.nblocks = (int32_t)mk::cdiv(s.N, BN);
+    s.base = base;
+    base += (int64_t)s.mblocks * s.nblocks;
+    segs[i] = s;
+  }
+  MK_REQUIRE(base < INT32_MAX, MK_ERR_ARG, "too many tiles");
+  mk_rot_plan* p = new mk_rot_plan();
+  p->nseg = nseg;
+  p->mode = mode;
+  p->ntiles = base;
+  if (hipMalloc((void**)&p->d_segs, std::max(1, nseg) * sizeof(RotSeg)) != hipSuccess) {
+    delete p;
+    mk::set_error("rot plan: alloc failed");
+    return MK_ERR_ALLOC;
+  }
+  if (nseg && hipMemcpy(p->d_segs, segs.data(), nseg * sizeof(RotSeg), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(p->d_segs);
+    delete p;
+    mk::set_error("rot plan: upload failed");
+    return MK_ERR_HIP;
+  }
+  *out = p;
+  return MK_OK;
+}
+
+extern "C" int mk_rot_plan_exec(const mk_rot_plan* p, const float* b_base, float* c_base, void* stream) {
+  MK_REQUIRE(p && b_base && c_base, MK_ERR_ARG, "null argument");
+  if (p->ntiles == 0) return MK_OK;
+  int64_t grid = mk::cdiv(p->ntiles, 8) * 8;
+  if (p->mode == 1)
+    rot_bf16_kernel<6><<<(unsigned)grid, 256, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, p->ntiles, b_base,
+                                                                          c_base);
+  else
+    rot_bf16_kernel<3><<<(unsigned)grid, 256, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, p->ntiles, b_base,
+                                                                          c_base);
+  return mk::check_launch("rot_bf16_kernel");
+}
+
+extern "C" int mk_rot_plan_destroy(mk_rot_plan* p) {
+  if (p) {
+    if (p->d_segs) (void)hipFree(p->d_segs);
+    delete p;
+  }
+  return MK_OK;
+}
+
+extern "C" int mk_gemm_nn_f64(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const void* B,
+                              int b_is_f32, int64_t ldb, double* C, int64_t ldc, void* stream) {
+  MK_REQUIRE(M > 0 && N >= 0 && K > 0 && A && B && C, MK_ERR_ARG, "bad arguments");
+  if (N == 0) return MK_OK;
+  MK_REQUIRE(M < 65536 * TB, MK_ERR_ARG, "M too large");
+  dim3 grid((unsigned)mk::cdiv(N, TB), (unsigned)mk::cdiv(M, TB));
+  hipStream_t st = mk::as_stream(stream);
+  if (b_is_f32)
+    gemm_nn_f64<float><<<grid, 256, 0, st>>>((int)M, (int)N, (int)K, A, lda, (const float*)B, ldb, C, ldc);
+  else
+    gemm_nn_f64<double><<<grid, 256, 0, st>>>((int)M, (int)N, (int)K, A, lda, (const double*)B, ldb, C, ldc);
+  return mk::check_launch("gemm_nn_f64");
+}
+
+extern "C" int64_t mk_gram_workspace_bytes(int64_t M, int64_t n) {
+  int64_t chunk = 8192;
+  int64_t nchunk = std::max<int64_t>(1, mk::cdiv(n, chunk));
+  return nchunk * M * M * 8 + 256;
+}
+
+extern "C" int mk_gram_f64(int64_t M, int64_t n, const void* D, int d_is_f32, int64_t ldd, double* G, void* work,
+                           int64_t work_bytes, void* stream) {
+  MK_REQUIRE(M > 0 && n > 0 && D && G && work, MK_ERR_ARG, "bad arguments");
+  MK_REQUIRE(work_bytes >= mk_gram_workspace_bytes(M, n), MK_ERR_ARG, "workspace too small");
+  int64_t chunk = 8192;
+  int64_t nchunk = std::max<int64_t>(1, mk::cdiv(n, chunk));
+  MK_REQUIRE(nchunk < 65536, MK_ERR_ARG, "n too large");
+  dim3 grid((unsigned)mk::cdiv(M, TB), (unsigned)mk::cdiv(M, TB), (unsigned)nchunk);
+  hipStream_t st = mk::as_stream(stream);
+  if (d_is_f32)
+    gram_partial_f64<float><<<grid, 256, 0, st>>>((int)M, n, (const float*)D, ldd, chunk, (double*)work);
+  else
+    gram_partial_f64<double><<<grid, 256, 0, st>>>((int)M, n, (const double*)D, ldd, chunk, (double*)work);
+  int r = mk::check_launch("gram_partial_f64");
+  if (r) return r;
+  sum_partials_f64<<<mk::grid_for(M * M, 256), 256, 0, st>>>((const double*)work, nchunk, M * M, G);
+  return mk::check_launch("sum_partials_f64");
+}

@@ -66,11 +66,21 @@ _SIGS = {
     "mk_filt_plan_create": ([I, P, I, ctypes.POINTER(P)], I),
     "mk_filt_plan_set_weights": ([P, P, P], I),
     "mk_filt_plan_set_halve": ([P, I], I),
+    "mk_filt_plan_borrow_spectrum": ([P, I, P, I], I),
     "mk_filt_plan_exec_f32": ([P, P, P, P], I),
     "mk_filt_plan_exec_f64": ([P, P, P, P], I),
     "mk_filt_plan_info": ([P, P], I),
     "mk_filt_plan_geometry": ([P, P], I),
     "mk_filt_plan_destroy": ([P], I),
+    "mk_split_bf16_padded_dims": ([I64, I64, P, P], I64),
+    "mk_split_bf16_matrix": ([P, I64, I64, I64, P, P, P, P], I),
+    "mk_rot_plan_create": ([I, P, I, ctypes.POINTER(P)], I),
+    "mk_rot_plan_exec": ([P, P, P, P], I),
+    "mk_rot_plan_destroy": ([P], I),
+    "mk_gemm_nn_f64": ([I64, I64, I64, P, I64, P, I, I64, P, I64, P], I),
+    "mk_gram_workspace_bytes": ([I64, I64], I64),
+    "mk_gram_f64": ([I64, I64, P, I, I64, P, P, I64, P], I),
+    "mk_smooth_rows_f64": ([P, P, P, I64, I64, P, I32, I32, I32, P], I),
 }
 
 _fns = {}

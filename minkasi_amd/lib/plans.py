@@ -1,0 +1,153 @@
+"""Owning Python handles for the library's plan objects (tod2map, filter,
+rotation).  Each handle keeps the tensors its device descriptors point at
+alive and destroys the plan when collected."""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import mkgpu
+from .. import device as dev
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        self._keep = []
+
+    def close(self):
+        if self.h and self.h.value and self._destroy:
+            try:
+                mkgpu.check(self._destroy, self.h)
+            except Exception:
+                pass
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class T2MPlan(_Handle):
+    """Deterministic tod2map plan over a (batch) ipix array."""
+
+    _destroy = "mk_t2m_plan_destroy"
+
+    def __init__(self, ipix, nx, ny, tile=(64, 32), item_samples=None):
+        super().__init__()
+        n = int(ipix.numel())
+        if item_samples is None:
+            # enough items to fill 256 CUs several times, large enough that
+            # the partial-tile traffic stays a few percent of the samples
+            item_samples = int(min(32768, max(1024, (n // (256 * 8)) // 64 * 64)))
+        self.nx, self.ny, self.tile, self.item_samples = int(nx), int(ny), tuple(tile), item_samples
+        self.nsamp = n
+        mkgpu.check("mk_t2m_plan_create", dev.ptr(ipix, torch.int32, "ipix") if n else 0, n, self.nx, self.ny,
+                    int(tile[0]), int(tile[1]), int(item_samples), ctypes.byref(self.h), dev.stream_ptr())
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        mkgpu.check("mk_t2m_plan_info", self.h, out.ctypes.data)
+        keys = ["nsamp", "nitems", "ntiles_touched", "tile_x", "tile_y", "plan_bytes", "partial_bytes", "max_item"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def exec(self, map_t, dat, accumulate):
+        if map_t.numel() != self.nx * self.ny:
+            raise ValueError("map size %d does not match plan %dx%d" % (map_t.numel(), self.nx, self.ny))
+        if dat.numel() != self.nsamp:
+            raise ValueError("TOD has %d samples, plan expects %d" % (dat.numel(), self.nsamp))
+        if map_t.dtype != dat.dtype:
+            raise TypeError("map and TOD dtypes differ (%s vs %s)" % (map_t.dtype, dat.dtype))
+        name = "mk_t2m_plan_exec_f32" if map_t.dtype == torch.float32 else "mk_t2m_plan_exec_f64"
+        dptr = dev.ptr(dat, name="dat") if self.nsamp else dev.ptr(map_t)
+        mkgpu.check(name, self.h, dev.ptr(map_t, name="map"), dptr, int(bool(accumulate)), dev.stream_ptr())
+
+
+class FiltPlan(_Handle):
+    """Batched C W C filter (noise filter core)."""
+
+    _destroy = "mk_filt_plan_destroy"
+
+    def __init__(self, segs, elem_bytes=4):
+        """segs: list of (rows, n, in_off, in_stride, out_off, out_stride)."""
+        super().__init__()
+        self.segs = [tuple(int(v) for v in s) for s in segs]
+        self.elem_bytes = elem_bytes
+        desc = np.asarray(self.segs, dtype=np.int64).reshape(-1)
+        if desc.size == 0:
+            desc = np.zeros(6, dtype=np.int64)
+        mkgpu.check("mk_filt_plan_create", len(self.segs), desc.ctypes.data, int(elem_bytes), ctypes.byref(self.h))
+
+    def set_weights(self, wdct):
+        mkgpu.check("mk_filt_plan_set_weights", self.h, dev.ptr(wdct, torch.float64, "wdct"), dev.stream_ptr())
+
+    def set_halve(self, halve):
+        mkgpu.check("mk_filt_plan_set_halve", self.h, int(bool(halve)))
+
+    def borrow(self, seg, src, src_seg=0):
+        mkgpu.check("mk_filt_plan_borrow_spectrum", self.h, int(seg), src.h, int(src_seg))
+        self._keep.append(src)
+
+    def exec(self, x, out):
+        if self.elem_bytes == 4:
+            mkgpu.check("mk_filt_plan_exec_f32", self.h, dev.ptr(x, torch.float32, "in"),
+                        dev.ptr(out, torch.float32, "out"), dev.stream_ptr())
+        else:
+            mkgpu.check("mk_filt_plan_exec_f64", self.h, dev.ptr(x, torch.float64, "in"),
+                        dev.ptr(out, torch.float64, "out"), dev.stream_ptr())
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        mkgpu.check("mk_filt_plan_info", self.h, out.ctypes.data)
+        keys = ["nseg", "items13", "items2", "work_bytes", "spec_bytes", "elem_bytes", "lds13", "lds2"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def geometry(self):
+        out = np.zeros(4 * max(1, len(self.segs)), dtype=np.int64)
+        mkgpu.check("mk_filt_plan_geometry", self.h, out.ctypes.data)
+        return out.reshape(-1, 4)[: len(self.segs)]
+
+
+class SplitMatrix:
+    """hi/mid/lo bf16 planes of an FP64 matrix, padded for the MFMA rotation."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        M, K = a.shape
+        mp, ld = ctypes.c_int64(), ctypes.c_int64()
+        mkgpu.fn("mk_split_bf16_padded_dims")(M, K, ctypes.byref(mp), ctypes.byref(ld))
+        self.M, self.K, self.M_pad, self.lda = M, K, mp.value, ld.value
+        d = dev.device()
+        a_dev = torch.from_numpy(a).to(d)
+        self.hi = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
+        self.mid = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
+        self.lo = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
+        mkgpu.check("mk_split_bf16_matrix", a_dev.data_ptr(), M, K, K, self.hi.data_ptr(), self.mid.data_ptr(),
+                    self.lo.data_ptr(), dev.stream_ptr())
+        self._src = a_dev  # kept until the split kernel has run on the stream
+
+
+class RotPlan(_Handle):
+    """Batched split-BF16 rotation C = A.B over segments."""
+
+    _destroy = "mk_rot_plan_destroy"
+
+    def __init__(self, segs, mode=1):
+        """segs: list of (SplitMatrix, N, b_off, b_ld, c_off, c_ld).
+        mode 0: 3 bf16 products; mode 1: 6 products (about FP32 accuracy)."""
+        super().__init__()
+        rows = []
+        for sm, N, b_off, b_ld, c_off, c_ld in segs:
+            rows.append([sm.M, sm.K, int(N), sm.hi.data_ptr(), sm.mid.data_ptr(), sm.lo.data_ptr(), sm.lda,
+                         int(b_off), int(b_ld), int(c_off), int(c_ld)])
+            self._keep.append(sm)
+        desc = np.asarray(rows, dtype=np.int64).reshape(-1)
+        if desc.size == 0:
+            desc = np.zeros(11, dtype=np.int64)
+        self.mode = int(mode)
+        mkgpu.check("mk_rot_plan_create", len(rows), desc.ctypes.data, self.mode, ctypes.byref(self.h))
+
+    def exec(self, b, c):
+        mkgpu.check("mk_rot_plan_exec", self.h, dev.ptr(b, torch.float32, "B"), dev.ptr(c, torch.float32, "C"),
+                    dev.stream_ptr())

@@ -1,0 +1,137 @@
+"""The batched operator A.x = P^T N^-1 P x over every TOD of a TodVec.
+
+The reference loops over TODs and, per TOD, calls map2tod, apply_noise and
+tod2map one after another (minkasi/tods/core.py:649-691, :1012-1020).  Here
+all TODs live in one arena (concatenated row-major ndet x n blocks) and each
+stage is ONE launch over the whole arena:
+
+  1. map2tod      flat gather over the arena                 (mk_map2tod_f32)
+  2. v.d          split-BF16 MFMA, one segment per TOD       (mk_rot_plan_exec)
+  3. C W C        four-step FFT filter, one segment per TOD  (mk_filt_plan_exec_f32)
+                  (edge halving folded in)
+  4. vT.(.)       split-BF16 MFMA                            (mk_rot_plan_exec)
+  5. tod2map      deterministic plan over the arena          (mk_t2m_plan_exec_f32)
+
+The sequence can be captured once in a HIP graph and replayed.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .. import device as dev
+from ..lib.plans import FiltPlan, RotPlan, T2MPlan
+from .map2tod import map2tod
+from .noise import NoiseSmoothedSVD
+
+
+class BatchedDot:
+    @staticmethod
+    def eligible(todvec, mapset):
+        from ..maps.skymap import SkyMap
+        if len(mapset.maps) != 1 or not isinstance(mapset.maps[0], SkyMap):
+            return False
+        if mapset.maps[0].map.dtype != torch.float32 or todvec.ntod == 0:
+            return False
+        for tod in todvec.tods:
+            if tod.noise_delayed or not isinstance(tod.noise, NoiseSmoothedSVD):
+                return False
+            if tod.noise.dtype != torch.float32 and tod.noise.filt32 is None:
+                return False
+        return True
+
+    def __init__(self, todvec, skymap, tile=(64, 32), item_samples=None):
+        self.tods = list(todvec.tods)
+        self.nx, self.ny, self.tag = skymap.nx, skymap.ny, skymap.tag
+        d = dev.device()
+        offs, sizes = [], []
+        off = 0
+        for tod in self.tods:
+            ndet, n = tod.noise.ndet, tod.noise.n
+            if (ndet, n) != tod.get_data_dims():
+                raise ValueError("noise model shape differs from TOD data")
+            offs.append(off)
+            sizes.append((ndet, n))
+            off += ndet * n
+        self.offs, self.sizes, self.nsamp = offs, sizes, off
+        # ipix arena; the TODs' cached pixelizations become views into it
+        self.ipix = torch.empty(max(off, 4), dtype=torch.int32, device=d)
+        for tod, o, (ndet, n) in zip(self.tods, offs, sizes):
+            ip = skymap.get_pix(tod)
+            self.ipix[o:o + ndet * n].copy_(ip.reshape(-1))
+            tod.info[self.tag] = self.ipix[o:o + ndet * n].view(ndet, n)
+        self.ipix_ids = [id(tod.info[self.tag]) for tod in self.tods]
+        self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
+        self.buf_a = torch.empty(max(off, 4), dtype=torch.float32, device=d)
+        self.buf_b = torch.empty(max(off, 4), dtype=torch.float32, device=d)
+        segs = [(ndet, n, o, n, o, n) for o, (ndet, n) in zip(offs, sizes)]
+        self.filt = FiltPlan(segs, 4)
+        for i, tod in enumerate(self.tods):
+            self.filt.borrow(i, tod.noise.filt32, 0)
+        self.filt.set_halve(True)
+        self.rot_v = RotPlan([(tod.noise.v_split, n, o, n, o, n) for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
+        self.rot_vT = RotPlan([(tod.noise.vT_split, n, o, n, o, n)
+                               for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
+        self.graph = None
+        self._gx = self._gout = None
+
+    def matches(self, todvec, mapset):
+        m = mapset.maps[0]
+        return (len(todvec.tods) == len(self.tods) and all(a is b for a, b in zip(todvec.tods, self.tods))
+                and m.nx == self.nx and m.ny == self.ny)
+
+    # ------------------------------------------------------------------
+    def stages(self, x, out, accumulate):
+        n = self.nsamp
+        a, b = self.buf_a[:n], self.buf_b[:n]
+        map2tod(a, x.reshape(-1), self.ipix[:n], do_add=False)
+        self.rot_v.exec(a, b)
+        self.filt.exec(b, b)
+        self.rot_vT.exec(b, a)
+        self.t2m.exec(out.reshape(-1), a, accumulate)
+
+    def apply(self, x, out, accumulate=True):
+        """out (+)= A x for device maps x, out (nx, ny) FP32."""
+        if self.graph is not None and x is self._gx and out is self._gout and accumulate == self._gacc:
+            self.graph.replay()
+            return out
+        self.stages(x, out, accumulate)
+        return out
+
+    def capture(self, x, out, accumulate=False, warmup=1):
+        """Captures apply(x, out) into a HIP graph replayed by later calls
+        with the same tensors."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self.stages(x, out, accumulate)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.stages(x, out, accumulate)
+        self.graph, self._gx, self._gout, self._gacc = g, x, out, accumulate
+        return g
+
+    def stage_times(self, x, out, reps=3):
+        """Per-stage device time (ms) with events on the launch stream."""
+        n = self.nsamp
+        a, b = self.buf_a[:n], self.buf_b[:n]
+        steps = [
+            ("map2tod", lambda: map2tod(a, x.reshape(-1), self.ipix[:n], do_add=False)),
+            ("rot_v", lambda: self.rot_v.exec(a, b)),
+            ("filter", lambda: self.filt.exec(b, b)),
+            ("rot_vT", lambda: self.rot_vT.exec(b, a)),
+            ("tod2map", lambda: self.t2m.exec(out.reshape(-1), a, False)),
+        ]
+        res = {}
+        for name, fn in steps:
+            fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            e1.synchronize()
+            res[name] = e0.elapsed_time(e1) / reps
+        return res

@@ -1,0 +1,227 @@
+"""Preconditioned conjugate gradient (minkasi/mapmaking/pcg.py:17-255).
+
+Same update order as the reference (pcg.py:101-177).  When b and x0 are a
+single FP32 SkyMap and the TodVec has a batched operator, the iteration
+runs with device-resident FP64 scalars (alpha, beta, r.z and r.r formed on
+the GPU by fused kernels); otherwise it runs the reference's Mapset-level
+loop with device vector operations.
+
+Additions (opt-in keywords): tol stops when ||r||/||b|| < tol (recursive
+residual; the reference always runs maxiter iterations);
+true_residual_every=k also records ||b - A x||/||b|| every k iterations;
+history (a list) receives one dict per iteration; verbose=False silences
+the per-iteration print.
+"""
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .. import device as dev
+from ..lib import mkgpu
+
+try:
+    from typing import Protocol, runtime_checkable
+except ImportError:  # pragma: no cover
+    from typing_extensions import Protocol, runtime_checkable
+
+myrank = 0
+
+
+@runtime_checkable
+class HasPrior(Protocol):
+    def apply_prior(self, x, Ax): ...
+
+
+class null_precon:
+    """pcg.py:241-255."""
+
+    def __init__(self):
+        self.isnull = True
+
+    def __add__(self, val):
+        return val
+
+    def __mul__(self, val):
+        return val
+
+
+class PCGBreakdown(RuntimeError):
+    """Zero/negative curvature (p.Ap <= 0) or a non-finite scalar."""
+
+
+def _single_f32_map(ms):
+    from ..maps.skymap import SkyMap
+    return (hasattr(ms, "maps") and len(ms.maps) == 1 and isinstance(ms.maps[0], SkyMap)
+            and ms.maps[0].map.dtype == torch.float32)
+
+
+def run_pcg_wprior(b, x0, tods, prior=None, precon=None, mask=None, maxiter=25, outroot="map", save_iters=[-1],
+                   save_ind=0, save_tail=".fits", plot_iters=[], plot_info=None, plot_ind=0, tol=None,
+                   true_residual_every=0, history=None, verbose=True):
+    fast = (prior is None and mask is None and _single_f32_map(b) and _single_f32_map(x0)
+            and tods.batched(b) is not None
+            and (precon is None or _single_f32_map(precon) or hasattr(precon, "apply_to")))
+    if fast:
+        return _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
+                               true_residual_every, history, verbose)
+    return _run_pcg_generic(b, x0, tods, prior, precon, mask, maxiter, outroot, save_iters, save_ind, save_tail,
+                            tol, history, verbose)
+
+
+def run_pcg(b, x0, tods, precon=None, mask=None, maxiter=25, outroot="map", save_iters=[-1], save_ind=0,
+            save_tail=".fits", plot_iters=[], plot_info=None, plot_ind=0, **kw):
+    """pcg.py:203-238."""
+    return run_pcg_wprior(b=b, x0=x0, tods=tods, prior=None, precon=precon, mask=mask, maxiter=maxiter,
+                          outroot=outroot, save_iters=save_iters, save_ind=save_ind, save_tail=save_tail,
+                          plot_iters=plot_iters, plot_info=plot_info, plot_ind=plot_ind, **kw)
+
+
+def _run_pcg_generic(b, x0, tods, prior, precon, mask, maxiter, outroot, save_iters, save_ind, save_tail, tol,
+                     history, verbose):
+    """pcg.py:101-177 line for line over Mapset operations."""
+    t1 = time.time()
+    Ax = tods.dot(x0, mask=mask)
+    try:
+        r = b.copy()
+        r.axpy(Ax, -1)
+    except Exception:
+        r = b - Ax
+    z = precon * r if precon is not None else r.copy()
+    p = z.copy()
+    zr = r.dot(z)
+    x = x0.copy()
+    bb = b.dot(b) if tol is not None else None
+    t2 = time.time()
+    t3 = 0
+    alpha = 0
+    nsamp = tods.get_nsamp()
+    tloop = time.time()
+    for i in range(maxiter):
+        if verbose and myrank == 0:
+            if i > 0:
+                print(i, zr, alpha, t2 - t1, t3 - t2, t3 - t1, nsamp / (t2 - t1) / 1e6)
+            else:
+                print(i, zr, t2 - t1)
+            sys.stdout.flush()
+        t1 = time.time()
+        Ap = tods.dot(p, mask=mask)
+        if prior is not None:
+            prior.apply_prior(p, Ap)
+        t2 = time.time()
+        pAp = p.dot(Ap)
+        if not (pAp > 0) or not np.isfinite(pAp):
+            raise PCGBreakdown("p.Ap = %r at iteration %d" % (pAp, i))
+        alpha = zr / pAp
+        x_new = x.copy()
+        x_new.axpy(p, alpha)
+        r_new = r.copy()
+        r_new.axpy(Ap, -alpha)
+        z_new = precon * r_new if precon is not None else r_new.copy()
+        zr_new = r_new.dot(z_new)
+        beta = zr_new / zr
+        p_new = z_new.copy()
+        p_new.axpy(p, beta)
+        p, z, r, zr, x = p_new, z_new, r_new, zr_new, x_new
+        t3 = time.time()
+        rel = None
+        if tol is not None or history is not None:
+            rr = r.dot(r)
+            rel = float(np.sqrt(rr / bb)) if bb else None
+        if history is not None:
+            history.append(dict(it=i, zr=zr, alpha=alpha, beta=beta, rel_res=rel, t_Ap=t2 - t1, t_iter=t3 - t1))
+        if i in save_iters and myrank == 0:
+            x.maps[save_ind].write(outroot + "_" + str(i) + save_tail)
+        if tol is not None and rel is not None and rel < tol:
+            break
+    tave = (time.time() - tloop) / max(1, i + 1) if maxiter else 0.0
+    if verbose and maxiter:
+        print("average time per iteration was ", tave, " with effective throughput ", nsamp / tave / 1e6,
+              " Msamp/s")
+    return x
+
+
+def _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
+                    true_residual_every, history, verbose):
+    """Device-scalar PCG with the same update order as pcg.py:143-170."""
+    op = tods.batched(b)
+    S = dev.scratch()
+    sp = dev.stream_ptr
+    bmap = b.maps[0].map
+    n = bmap.numel()
+    x = x0.copy()
+    xm = x.maps[0].map
+    r = torch.empty_like(bmap)
+    z = torch.empty_like(bmap)
+    p = torch.empty_like(bmap)
+    Ap = torch.empty_like(bmap)
+    sc = S.sc
+    sc.zero_()
+    work = S.work.data_ptr()
+    jacobi = precon is not None and _single_f32_map(precon)
+    w = precon.maps[0].map if jacobi else None
+
+    def apply_precon(rv, zv):
+        if precon is None:
+            zv.copy_(rv)
+        elif jacobi:
+            mkgpu.check("mk_mul_f32", zv.data_ptr(), w.data_ptr(), rv.data_ptr(), n, sp())
+        else:
+            precon.apply_to(rv, zv)
+
+    t1 = time.time()
+    op.apply(xm, Ap, accumulate=False)
+    r.copy_(bmap)
+    mkgpu.check("mk_axpy_f32", r.data_ptr(), Ap.data_ptr(), -1.0, n, sp())
+    apply_precon(r, z)
+    mkgpu.check("mk_pcg_init", r.data_ptr(), z.data_ptr(), p.data_ptr(), n, sc.data_ptr(), work, sp())
+    mkgpu.check("mk_dot_f32", bmap.data_ptr(), bmap.data_ptr(), n, sc.data_ptr(), work, sp())
+    bb = float(sc[mkgpu.SC_DOT].item())
+    nsamp = tods.get_nsamp()
+    tloop = time.time()
+    it_done = 0
+    for i in range(maxiter):
+        t1 = time.time()
+        op.apply(p, Ap, accumulate=False)
+        mkgpu.check("mk_pcg_alpha", p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(), work, sp())
+        if jacobi:
+            mkgpu.check("mk_pcg_xrz_jacobi", xm.data_ptr(), r.data_ptr(), z.data_ptr(), p.data_ptr(), Ap.data_ptr(),
+                        w.data_ptr(), n, sc.data_ptr(), work, sp())
+        else:
+            mkgpu.check("mk_pcg_xr", xm.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(),
+                        work, sp())
+            apply_precon(r, z)
+            mkgpu.check("mk_pcg_beta", r.data_ptr(), z.data_ptr(), n, sc.data_ptr(), work, sp())
+        mkgpu.check("mk_pcg_p", p.data_ptr(), z.data_ptr(), n, sc.data_ptr(), sp())
+        it_done = i + 1
+        need_host = verbose or tol is not None or history is not None
+        if need_host:
+            s = sc.cpu().numpy()
+            t3 = time.time()
+            if s[mkgpu.SC_FLAG] != 0:
+                raise PCGBreakdown("PCG breakdown at iteration %d (flag %g, pAp %r)" % (i, s[mkgpu.SC_FLAG],
+                                                                                      s[mkgpu.SC_PAP]))
+            rel = float(np.sqrt(s[mkgpu.SC_RR] / bb)) if bb > 0 else float("nan")
+            rec = dict(it=i, zr=float(s[mkgpu.SC_ZR]), alpha=float(s[mkgpu.SC_ALPHA]), beta=float(s[mkgpu.SC_BETA]),
+                       rel_res=rel, t_iter=t3 - t1)
+            if true_residual_every and (i + 1) % true_residual_every == 0:
+                rt = torch.empty_like(bmap)
+                op.apply(xm, rt, accumulate=False)
+                rt = bmap.to(torch.float64) - rt.to(torch.float64)
+                rec["true_rel_res"] = float(torch.linalg.vector_norm(rt).item() / np.sqrt(bb))
+            if history is not None:
+                history.append(rec)
+            if verbose and myrank == 0:
+                print(i, rec["zr"], rec["alpha"], rec["t_iter"], nsamp / max(rec["t_iter"], 1e-12) / 1e6)
+                sys.stdout.flush()
+            if tol is not None and rel < tol:
+                break
+        if i in save_iters and myrank == 0:
+            x.maps[save_ind].write(outroot + "_" + str(i) + save_tail)
+    torch.cuda.synchronize()
+    if verbose and it_done:
+        tave = (time.time() - tloop) / it_done
+        print("average time per iteration was ", tave, " with effective throughput ", nsamp / tave / 1e6,
+              " Msamp/s")
+    return x

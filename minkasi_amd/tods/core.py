@@ -1,0 +1,293 @@
+"""Tod and TodVec with the reference's interface (minkasi/tods/core.py:30-1047).
+
+TOD arrays in ``info`` may be numpy arrays or device tensors; projections,
+noise and dot products run on the device.  ``TodVec.dot`` sends every TOD
+through one batched operator (mapmaking/batch.py: one launch per stage over
+all TODs) when the mapset is a single FP32 SkyMap and every TOD has a
+NoiseSmoothedSVD; otherwise it follows the reference's per-TOD loop
+(core.py:1012-1020) with device kernels.
+"""
+import copy
+import time
+
+import numpy as np
+import torch
+
+from .. import device as dev
+from ..mapmaking.noise import NoiseSmoothedSVD, WithDetWeights
+
+
+def _torch_dtype(a):
+    if isinstance(a, torch.Tensor):
+        return a.dtype
+    return {np.dtype(np.float32): torch.float32}.get(np.asarray(a).dtype, torch.float64)
+
+
+class Tod:
+    def __init__(self, info):
+        self.info = info.copy()
+        self.jumps = None
+        self.cuts = None
+        self.noise = None
+        self.noise_delayed = False
+        self.noise_args = ()
+        self.noise_kwargs = {}
+
+    # ---------------------------------------------------------------- shape
+    def lims(self):
+        if "dx" not in self.info and "ra_b" in self.info:
+            # separable raster: min/max of a sum are the sums of min/max
+            # (rounding is monotone), so this equals the materialised result
+            i = self.info
+            return [float(np.min(i["ra_b"]) + np.min(i["dra"])), float(np.max(i["ra_b"]) + np.max(i["dra"])),
+                    float(np.min(i["dec_b"]) + np.min(i["ddec"])), float(np.max(i["dec_b"]) + np.max(i["ddec"]))]
+        dx, dy = self.info["dx"], self.info["dy"]
+        f = (lambda a: float(a.min()), lambda a: float(a.max()))
+        return [f[0](dx), f[1](dx), f[0](dy), f[1](dy)]
+
+    def get_ndet(self):
+        return int(self.info["dat_calib"].shape[0])
+
+    def get_ndata(self):
+        return int(self.info["dat_calib"].shape[1])
+
+    def get_nsamp(self):
+        return self.get_ndet() * self.get_ndata()
+
+    def get_data_dims(self):
+        return (self.get_ndet(), self.get_ndata())
+
+    # ---------------------------------------------------------------- pixelization
+    def get_saved_pix(self, tag="ipix"):
+        return self.info.get(tag, None)
+
+    def clear_saved_pix(self, tag="ipix"):
+        if tag in self.info.keys():
+            del self.info[tag]
+
+    def save_pixellization(self, tag, ipix):
+        if tag in self.info.keys():
+            print("warning - overwriting key ", tag, " in tod.save_pixellization.")
+        self.info[tag] = ipix
+
+    def set_tag(self, tag):
+        if "tag" in self.info:
+            print("Warning: overwriting tag")
+        self.info["tag"] = tag
+
+    def set_pix(self, map):
+        ipix = map.get_pix(self)
+        self.save_pixellization(map.tag, ipix)
+
+    # ---------------------------------------------------------------- data
+    def get_data(self, dat=None):
+        if dat is None:
+            dat = self.info.get("dat_calib", None)
+        if dat is None:
+            raise ValueError("This TOD has no stored data. Populare tod.info['dat_calib']")
+        return dat
+
+    def get_tvec(self):
+        return self.info["ctime"]
+
+    def get_radec(self):
+        return self.info["dx"], self.info["dy"]
+
+    def get_empty(self, clear=False):
+        """Device tensor (ndet, ndata) in the dtype of info['dtype'] or of
+        dat_calib (core.py:286-311)."""
+        if "dtype" in self.info.keys():
+            dtype = _torch_dtype(np.zeros(1, dtype=self.info["dtype"]))
+        elif "dat_calib" in self.info.keys():
+            dtype = _torch_dtype(self.info["dat_calib"])
+        else:
+            dtype = torch.float64
+        f = torch.zeros if clear else torch.empty
+        return f((self.get_ndet(), self.get_ndata()), dtype=dtype, device=dev.device())
+
+    def copy(self, copy_info=False):
+        if copy_info:
+            myinfo = self.info.copy()
+            for key in myinfo.keys():
+                try:
+                    myinfo[key] = self.info[key].clone() if isinstance(self.info[key], torch.Tensor) \
+                        else self.info[key].copy()
+                except Exception:
+                    pass
+            tod = Tod(myinfo)
+        else:
+            tod = Tod(self.info)
+        if self.jumps is not None:
+            tod.jumps = list(self.jumps)
+        if self.cuts is not None:
+            tod.cuts = list(self.cuts)
+        tod.noise = self.noise
+        tod.noise_delayed = self.noise_delayed
+        tod.noise_args, tod.noise_kwargs = self.noise_args, self.noise_kwargs
+        if hasattr(self, "noise_modelclass"):
+            tod.noise_modelclass = self.noise_modelclass
+        return tod
+
+    # ---------------------------------------------------------------- noise
+    def set_noise(self, modelclass=NoiseSmoothedSVD, dat=None, delayed=False, *args, **kwargs):
+        if delayed:
+            self.noise_args = copy.deepcopy(args)
+            self.noise_kwargs = copy.deepcopy(kwargs)
+            self.noise_delayed = True
+            self.noise_modelclass = modelclass
+        else:
+            self.noise_delayed = False
+            dat = self.get_data(dat)
+            self.noise = modelclass(dat, *args, **kwargs)
+
+    def get_det_weights(self):
+        if self.noise is None:
+            print("noise model not set in get_det_weights.")
+            return None
+        if isinstance(self.noise, WithDetWeights):
+            return self.noise.get_det_weights()
+        print("noise model does not support detector weights in get_det_weights.")
+        return None
+
+    def apply_noise(self, dat=None):
+        """core.py:517-552.  The reference silently falls back to a hard-wired
+        model when the noise object raises; here the error propagates."""
+        dat = self.get_data(dat)
+        if self.noise_delayed:
+            self.noise = self.noise_modelclass(dat, *(self.noise_args), **(self.noise_kwargs))
+            self.noise_delayed = False
+        if self.noise is None:
+            raise RuntimeError("tod.noise is not set; call set_noise first")
+        return self.noise.apply_noise(dat)
+
+    # ---------------------------------------------------------------- projections
+    def mapset2tod(self, mapset, dat=None):
+        if dat is None:
+            dat = self.get_empty(True)
+        for map in mapset.maps:
+            map.map2tod(self, dat)
+        return dat
+
+    def tod2mapset(self, mapset, dat=None):
+        if dat is None:
+            dat = self.get_data()
+        for map in mapset.maps:
+            map.tod2map(self, dat)
+
+    def dot(self, mapset, mapset_out, times=False, mask=None):
+        """core.py:649-691 (mask applied in place to the input, as the reference)."""
+        t1 = time.time()
+        if mask is not None:
+            for i, imap in enumerate(mapset.maps):
+                imap.map *= mask.maps[i].map
+        tmp = self.mapset2tod(mapset)
+        t2 = time.time()
+        tmp = self.apply_noise(tmp)
+        t3 = time.time()
+        self.tod2mapset(mapset_out, tmp)
+        if mask is not None:
+            for i, imap in enumerate(mapset_out.maps):
+                imap.map *= mask.maps[i].map
+        t4 = time.time()
+        if times:
+            return np.asarray([t2 - t1, t3 - t2, t4 - t3])
+
+    def iter_init(self):
+        return
+
+    def iter_finalize(self):
+        return
+
+    def timestream_chisq(self, dat=None):
+        dat = self.get_data(dat)
+        d = dev.to_device(dat)
+        filt = self.apply_noise(d)
+        return float((filt.to(torch.float64) * d.to(torch.float64)).sum().item())
+
+
+class TodVec:
+    def __init__(self):
+        self.tods = []
+        self.ntod = 0
+        self._batch = None
+
+    def add_tod(self, tod, copy_info=False):
+        self.tods.append(tod.copy(copy_info))
+        self.tods[-1].set_tag(self.ntod)
+        self.ntod = self.ntod + 1
+        self._batch = None
+
+    def lims(self):
+        if self.ntod == 0:
+            return None
+        xmin, xmax, ymin, ymax = self.tods[0].lims()
+        for i in range(1, self.ntod):
+            x1, x2, y1, y2 = self.tods[i].lims()
+            xmin = min(x1, xmin)
+            xmax = max(x2, xmax)
+            ymin = min(y1, ymin)
+            ymax = max(y2, ymax)
+        return [xmin, xmax, ymin, ymax]
+
+    def get_nsamp(self, reduce=True):
+        tot = 0
+        for tod in self.tods:
+            tot = tot + tod.get_nsamp()
+        return tot
+
+    def set_pix(self, map):
+        for tod in self.tods:
+            tod.set_pix(map)
+
+    # ---------------------------------------------------------------- A.x
+    def batched(self, mapset, **kw):
+        """The batched operator for this TodVec and mapset geometry (built once,
+        cached).  None when the inputs do not fit the batched path."""
+        from ..mapmaking.batch import BatchedDot
+        if not BatchedDot.eligible(self, mapset):
+            return None
+        b = self._batch
+        if b is None or not b.matches(self, mapset):
+            b = BatchedDot(self, mapset.maps[0], **kw)
+            self._batch = b
+        return b
+
+    def dot_cached(self, mapset, mapset_out=None, mask=None):
+        return self.dot(mapset, mapset_out, mask=mask)
+
+    def dot(self, mapset, mapset_out=None, report_times=False, cache_maps=False, mask=None):
+        """core.py:967-1026."""
+        if mapset_out is None:
+            mapset_out = mapset.copy()
+            mapset_out.clear()
+        b = None if report_times else self.batched(mapset)
+        if b is not None:
+            if mask is not None:
+                mapset.maps[0].map *= mask.maps[0].map
+            b.apply(mapset.maps[0].map, mapset_out.maps[0].map, accumulate=True)
+            if mask is not None:
+                mapset_out.maps[0].map *= mask.maps[0].map
+            return mapset_out
+        times = np.zeros(self.ntod)
+        tot_times = np.zeros(3)
+        for i in range(self.ntod):
+            tod = self.tods[i]
+            t1 = time.time()
+            tod.iter_init()
+            mytimes = tod.dot(mapset, mapset_out, True, mask=mask)
+            tod.iter_finalize()
+            t2 = time.time()
+            tot_times = tot_times + mytimes
+            times[i] = t2 - t1
+        if report_times:
+            return mapset_out, times
+        return mapset_out
+
+    def make_rhs(self, mapset, do_clear=False):
+        """core.py:1028-1047."""
+        if do_clear:
+            mapset.clear()
+        for tod in self.tods:
+            dat_filt = tod.apply_noise()
+            for map in mapset.maps:
+                map.tod2map(tod, dat_filt)
