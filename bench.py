@@ -1,0 +1,211 @@
+"""Benchmark: batched PCG mapmaking on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4]
+
+Workload (default): 64 synthetic TODs, detector counts uniform in 64-512 and
+lengths uniform in 10000-200000 samples, FP32, on a 2048 x 2048 map at
+1 arcsec, NoiseSmoothedSVD noise (fwhm 50 bins), hits-count Jacobi PCG.
+A step is one PCG iteration over all TODs: the batched A.p (map2tod, split-
+BF16 MFMA rotation, FFT noise filter, rotation back, deterministic tod2map,
+captured in one HIP graph) plus the fused FP64-scalar vector updates.
+
+Prints ONE JSON line: value = PCG iterations per second (whole job), plus
+the reference's own throughput definition (Msamp/s = sum ndet*n / seconds
+per iteration, minkasi/mapmaking/pcg.py:191-198), time to ||r||/||b|| < tol
+for the solvers, a roofline entry for the dominant kernel and a CPU baseline
+measured with the FP64 oracle on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+METRIC = "time-to-‖r‖/‖b‖<1e-6 & PCG iters/s, learned vs Jacobi PCG, 64 TODs"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--ntod", type=int, default=None)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--maxiter", type=int, default=600, help="cap for the solve-to-tolerance runs")
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance runs")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3: setup + steps only")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(P):
+    """Algorithmic HBM bytes per launch of each stage (FP32 TOD, int32 ipix)."""
+    S = P.nsamp
+    npix = P.map.nx * P.map.ny
+    return {
+        "map2tod": 8 * S + 4 * npix,      # ipix in, TOD out (+ map once)
+        "rot_v": 8 * S,                    # TOD in, rotated TOD out
+        "filter": 12 * S,                  # rotated TOD in, mywt in, filtered out
+        "rot_vT": 8 * S,
+        "tod2map": 8 * S + 4 * npix,      # ipix in, TOD in (+ map once)
+    }
+
+
+def cpu_baseline(P, budget_s=20.0):
+    """FP64 oracle (OpenMP projections, numpy GEMM, scipy DCT-I) on one TOD of
+    the workload's median size; scaled to PCG iterations/s of the full job."""
+    import torch
+    from oracle import core, noise as onoise, solve
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    import scipy.fft
+    shapes = sorted(P.shapes, key=lambda s: s[0] * s[1])
+    ndet, n = shapes[len(shapes) // 2]
+    k = [i for i, s in enumerate(P.shapes) if s == (ndet, n)][0]
+    tod = P.todvec.tods[k]
+    d = tod.get_data()
+    d = d.cpu().numpy().astype(np.float64) if isinstance(d, torch.Tensor) else np.asarray(d, np.float64)
+    ipix = tod.info[P.map.tag].cpu().numpy().reshape(ndet, n)
+    with scipy.fft.set_workers(cores):
+        t0 = time.time()
+        nz = onoise.NoiseSmoothedSVD(d, fwhm=50.0)
+        t_init = time.time() - t0
+        otod = solve.OTod(d, ipix, nz)
+        x = np.random.default_rng(0).standard_normal((P.map.nx, P.map.ny))
+        reps, t_ax = 0, 0.0
+        while reps < 1 or (t_ax < budget_s and reps < 5):
+            t0 = time.time()
+            solve.todvec_dot([otod], x)
+            t_ax += time.time() - t0
+            reps += 1
+    per_ax = t_ax / reps
+    scale = P.nsamp / float(ndet * n)
+    return {
+        "value": 1.0 / (per_ax * scale),
+        "unit": "PCG iter/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "FP64 oracle A.x on one TOD of %d x %d samples (%d reps, %.2f s each; noise init %.1f s), "
+                  "scaled by total/sample samples = %.1f; FFTW unavailable, scipy DCT-I" % (
+                      ndet, n, reps, per_ax, t_init, scale),
+        "msamp_per_s": ndet * n / per_ax / 1e6,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the batched solver holds every TOD on one device "
+              "(multi-GPU sharding over TODs is described in DESIGN.md but not built).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from minkasi_amd.mapmaking.pcg import DevicePCG
+    from minkasi_amd.workloads import build_problem
+    from minkasi_amd.synth import CONFIGS
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t_setup = time.time()
+    P = build_problem(args.config, ntod=args.ntod, verbose=True)
+    log("setup: generate %.1fs pixelize %.1fs noise %.1fs plans %.1fs rhs %.1fs; %d TODs, %.3g samples" % (
+        P.t_generate, P.t_pixelize, P.t_noise, P.t_plans, P.t_rhs, len(P.shapes), P.nsamp))
+    op = P.op
+    b = P.rhs.maps[0].map
+    # one graph for A.p on the solver's own buffers
+    solver = DevicePCG(op, b, P.precon)
+    x = torch.zeros_like(b)
+    solver.init(x)
+    if not args.no_graph:
+        op.capture(solver.p, solver.Ap, accumulate=False)
+    t_setup = time.time() - t_setup
+
+    # ---- timed PCG steps (Jacobi)
+    for _ in range(args.warmup):
+        solver.step()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(args.steps):
+        solver.step()
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    ms_per_step = 1000.0 * dt / args.steps
+    iters_per_s = args.steps / dt
+    rel_after = solver.rel_res()
+    log("PCG: %.2f ms/iter, %.2f iter/s, %.0f Msamp/s; rel res after %d its %.3e" % (
+        ms_per_step, iters_per_s, P.nsamp * iters_per_s / 1e6, args.warmup + args.steps, rel_after))
+
+    # ---- per-stage times (events on the launch stream) and roofline
+    st = op.stage_times(solver.p, torch.empty_like(b), reps=3)
+    sb = stage_bytes(P)
+    dom = max(st, key=st.get)
+    achieved = sb[dom] / (st[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": int(sb[dom]), "stage_ms": {k: round(v, 3) for k, v in st.items()},
+            "stage_frac": {k: round(sb[k] / (st[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k in st}}
+    log("stage ms:", json.dumps(roof["stage_ms"]), "frac:", json.dumps(roof["stage_frac"]))
+
+    result = {
+        "metric": METRIC, "value": round(iters_per_s, 4), "unit": "PCG iter/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "config%d: %d TODs, ndet %s, n %s, %dx%d map at %g arcsec, NoiseSmoothedSVD, "
+                               "Jacobi PCG" % (args.config, len(P.shapes), CONFIGS[args.config]["ndet"],
+                                               CONFIGS[args.config]["n"], P.map.nx, P.map.ny,
+                                               CONFIGS[args.config]["pix"]),
+                   "ntod": len(P.shapes), "nsamp": int(P.nsamp), "map": [P.map.nx, P.map.ny]},
+        "msamp_per_s": round(P.nsamp * iters_per_s / 1e6, 1),
+        "setup_s": round(t_setup, 1),
+        "roofline": roof,
+    }
+    if args.profile:
+        print(json.dumps(result))
+        return
+
+    # ---- time to tolerance (recursive residual), Jacobi
+    if not args.no_solve:
+        res = {}
+        solver2 = DevicePCG(op, b, P.precon)
+        x2 = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        solver2.init(x2)
+        it = 0
+        rel = 1.0
+        while it < args.maxiter:
+            solver2.step()
+            it += 1
+            rel = solver2.rel_res()
+            if rel < args.tol:
+                break
+        torch.cuda.synchronize()
+        tj = time.time() - t0
+        res["jacobi"] = {"iters": it, "seconds": round(tj, 3), "rel_res": rel, "reached": bool(rel < args.tol),
+                         "true_rel_res": solver2.true_rel_res()}
+        log("Jacobi to tol:", json.dumps(res["jacobi"]))
+        result["time_to_tol"] = res
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline(P)
+        except Exception as e:  # reported, not fatal
+            result["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -182,7 +182,10 @@ int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
  * desc (host): 6 int64 per segment (TOD): rows, n, in_off, in_stride,
  *   out_off, out_stride (element offsets into the exec in/out arrays).
  * elem_bytes: 4 (FP32 transform) or 8 (FP64 transform).
- * The plan owns its work buffer and spectra (allocated at creation). */
+ * The plan owns its spectra (allocated at creation) and a work buffer of
+ * rows*P complex values, allocated by the first call that needs it (so
+ * capture a graph only after one eager exec) and freed by
+ * mk_filt_plan_release_work. */
 typedef struct mk_filt_plan mk_filt_plan;
 int mk_filt_plan_create(int nseg, const int64_t *desc, int elem_bytes,
                         mk_filt_plan **plan);
@@ -201,6 +204,8 @@ int mk_filt_plan_borrow_spectrum(mk_filt_plan *plan, int seg,
 int mk_filt_plan_set_halve(mk_filt_plan *plan, int halve_edges);
 int mk_filt_plan_exec_f32(const mk_filt_plan *plan, const float *in,
                           float *out, void *stream);
+/* Frees the work buffer (synchronises the stream first). */
+int mk_filt_plan_release_work(mk_filt_plan *plan, void *stream);
 int mk_filt_plan_exec_f64(const mk_filt_plan *plan, const double *in,
                           double *out, void *stream);
 /* info[0..7] = nseg, K1/K3 items, K2 items, work bytes, spectrum bytes,

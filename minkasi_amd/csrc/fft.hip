@@ -32,6 +32,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 namespace {
@@ -56,112 +57,142 @@ __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.re * s, a.im *
 __device__ __forceinline__ void sincospi_t(float x, float* s, float* c) { sincospif(x, s, c); }
 __device__ __forceinline__ void sincospi_t(double x, double* s, double* c) { sincospi(x, s, c); }
 
-// exp(-i*pi*num/den) for integers 0 <= num < 2*den (den a power of two
-// makes num/den exact).
+// exp(-i*pi*num/den) for integers 0 <= num < 2*den.  expmi2 requires den a
+// power of two and num < 2^24, so -num * (1/den) is exact in T.
 template <typename T>
 __device__ __forceinline__ cx<T> expmi(int64_t num, int64_t den) {
   T s, c;
   sincospi_t((T)(-(double)num / (double)den), &s, &c);
   return {c, s};
 }
+template <typename T>
+__device__ __forceinline__ cx<T> expmi2(int64_t num, T inv_den) {
+  T s, c;
+  sincospi_t(-(T)num * inv_den, &s, &c);
+  return {c, s};
+}
 
-constexpr int kThreads = 256;
-constexpr int kEPT = 32;                       // complex elements per thread in an LDS FFT
-constexpr int kMaxWG = kThreads * kEPT;        // 8192 complex points per workgroup
+constexpr int kThreads = 512;                  // threads per FFT workgroup
+constexpr int kMaxWG = 4096;                   // complex points per workgroup in an LDS FFT (8 per thread)
 constexpr int kMaxLds = 144 * 1024;            // dynamic LDS budget per workgroup (gfx950 has 160 KB)
 
-// In-place batched LDS FFT: buf[b*bstride + e*estride], e < len = 2^l2len,
-// b < nbatch, nbatch*len <= kMaxWG.  tw[i] = exp(-2 pi i i/len).
+// LDS rows are padded: element e of a row lives at e + e/8 and rows are
+// row_stride(len) apart (odd), so the strided Stockham writes of the early
+// radix-8 stages, the row-parallel writes of K1/K3 and the unit-stride reads
+// all map to distinct banks.
+__host__ __device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
+__host__ __device__ __forceinline__ int row_stride(int len) { return len + (len >> 3) + 1; }
+
+// In-register DFTs (natural order in and out).
 template <typename T, bool INV>
-__device__ void lds_fft(cx<T>* buf, const cx<T>* tw, int l2len, int nbatch, int bstride, int estride) {
-  const int len = 1 << l2len;
-  const int tid = threadIdx.x;
-  int Ns = 1, lns = 0;
-  int rem = l2len;
+__device__ __forceinline__ void dft2(cx<T>* v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cx<T>& a0, cx<T>& a1, cx<T>& a2, cx<T>& a3) {
+  cx<T> b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3), d = csub(a1, a3);
+  cx<T> b3 = INV ? cx<T>{-d.im, d.re} : cx<T>{d.im, -d.re};  // (+/-i) * d
+  a0 = cadd(b0, b2);
+  a2 = csub(b0, b2);
+  a1 = cadd(b1, b3);
+  a3 = csub(b1, b3);
+}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4v(cx<T>* v) { dft4<T, INV>(v[0], v[1], v[2], v[3]); }
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+  const T c = T(0.70710678118654752440);
+  cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<T, INV>(e0, e1, e2, e3);
+  dft4<T, INV>(o0, o1, o2, o3);
+  if (!INV) {
+    o1 = {c * (o1.re + o1.im), c * (o1.im - o1.re)};   // * exp(-i pi/4)
+    o2 = {o2.im, -o2.re};                               // * -i
+    o3 = {c * (o3.im - o3.re), -c * (o3.re + o3.im)};  // * exp(-3i pi/4)
+  } else {
+    o1 = {c * (o1.re - o1.im), c * (o1.re + o1.im)};   // * exp(i pi/4)
+    o2 = {-o2.im, o2.re};                               // * i
+    o3 = {-c * (o3.re + o3.im), c * (o3.re - o3.im)};  // * exp(3i pi/4)
+  }
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+// One radix-R Stockham stage over nrows padded rows of length 2^l2len.
+// Every thread loads all of its butterflies' inputs, the workgroup
+// synchronises, then the outputs are written in place.
+template <typename T, bool INV, int LR>
+__device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* tw, int l2len, int nrows, int rs, int Ns,
+                                          int lns) {
+  constexpr int R = 1 << LR;
+  constexpr int NB = kMaxWG / R / kThreads;
+  const int lq = l2len - LR, q = 1 << lq;
+  const int nbf = nrows << lq;
+  cx<T> v[NB][R];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    int bf = threadIdx.x + u * kThreads;
+    if (bf < nbf) {
+      int row = bf >> lq, j = bf & (q - 1), k = j & (Ns - 1);
+      const cx<T>* base = buf + row * rs;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * q)];
+      int st = k << (l2len - lns - LR);  // (len / (R Ns)) * k
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        cx<T> w = tw[st * r];
+        if (INV) w = cconj(w);
+        v[u][r] = cmul(v[u][r], w);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    int bf = threadIdx.x + u * kThreads;
+    if (bf < nbf) {
+      int row = bf >> lq, j = bf & (q - 1), k = j & (Ns - 1);
+      if (LR == 3) dft8<T, INV>(v[u]);
+      else if (LR == 2) dft4v<T, INV>(v[u]);
+      else dft2<T, INV>(v[u]);
+      cx<T>* base = buf + row * rs;
+      int idx = ((j >> lns) << (lns + LR)) + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) base[pad8(idx + r * Ns)] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place FFT of nrows padded rows (row stride rs) of length 2^l2len,
+// nrows * 2^l2len <= kMaxWG; tw[i] = exp(-2 pi i i/len) (unpadded).
+template <typename T, bool INV>
+__device__ void lds_fft(cx<T>* buf, const cx<T>* tw, int l2len, int nrows, int rs) {
+  int Ns = 1, lns = 0, rem = l2len;
   while (rem > 0) {
-    if (rem >= 2) {
-      const int lq = l2len - 2, q = 1 << lq;
-      const int nbf = nbatch * q;
-      cx<T> v[kEPT];
-#pragma unroll
-      for (int u = 0; u < kEPT / 4; u++) {
-        int bf = tid + u * kThreads;
-        if (bf < nbf) {
-          int bt = bf >> lq, j = bf & (q - 1);
-          int k = j & (Ns - 1);
-          cx<T>* base = buf + bt * bstride;
-#pragma unroll
-          for (int r = 0; r < 4; r++) v[4 * u + r] = base[(j + r * q) * estride];
-          int st = k << (l2len - lns - 2);  // (len/(4 Ns)) * k
-#pragma unroll
-          for (int r = 1; r < 4; r++) {
-            cx<T> w = tw[st * r];
-            if (INV) w = cconj(w);
-            v[4 * u + r] = cmul(v[4 * u + r], w);
-          }
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kEPT / 4; u++) {
-        int bf = tid + u * kThreads;
-        if (bf < nbf) {
-          int bt = bf >> lq, j = bf & (q - 1);
-          int k = j & (Ns - 1);
-          cx<T> a0 = cadd(v[4 * u], v[4 * u + 2]), a1 = csub(v[4 * u], v[4 * u + 2]);
-          cx<T> a2 = cadd(v[4 * u + 1], v[4 * u + 3]), d = csub(v[4 * u + 1], v[4 * u + 3]);
-          cx<T> a3 = INV ? cx<T>{-d.im, d.re} : cx<T>{d.im, -d.re};  // (+/-i) * d
-          cx<T>* base = buf + bt * bstride;
-          int idx = ((j >> lns) << (lns + 2)) + k;
-          base[idx * estride] = cadd(a0, a2);
-          base[(idx + Ns) * estride] = cadd(a1, a3);
-          base[(idx + 2 * Ns) * estride] = csub(a0, a2);
-          base[(idx + 3 * Ns) * estride] = csub(a1, a3);
-        }
-      }
-      __syncthreads();
+    if (rem >= 3) {
+      fft_stage<T, INV, 3>(buf, tw, l2len, nrows, rs, Ns, lns);
+      Ns <<= 3; lns += 3; rem -= 3;
+    } else if (rem == 2) {
+      fft_stage<T, INV, 2>(buf, tw, l2len, nrows, rs, Ns, lns);
       Ns <<= 2; lns += 2; rem -= 2;
     } else {
-      const int lq = l2len - 1, q = 1 << lq;
-      const int nbf = nbatch * q;
-      cx<T> v[kEPT];
-#pragma unroll
-      for (int u = 0; u < kEPT / 2; u++) {
-        int bf = tid + u * kThreads;
-        if (bf < nbf) {
-          int bt = bf >> lq, j = bf & (q - 1);
-          int k = j & (Ns - 1);
-          cx<T>* base = buf + bt * bstride;
-          v[2 * u] = base[j * estride];
-          cx<T> w = tw[k << (l2len - lns - 1)];
-          if (INV) w = cconj(w);
-          v[2 * u + 1] = cmul(base[(j + q) * estride], w);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kEPT / 2; u++) {
-        int bf = tid + u * kThreads;
-        if (bf < nbf) {
-          int bt = bf >> lq, j = bf & (q - 1);
-          int k = j & (Ns - 1);
-          cx<T>* base = buf + bt * bstride;
-          int idx = ((j >> lns) << (lns + 1)) + k;
-          base[idx * estride] = cadd(v[2 * u], v[2 * u + 1]);
-          base[(idx + Ns) * estride] = csub(v[2 * u], v[2 * u + 1]);
-        }
-      }
-      __syncthreads();
+      fft_stage<T, INV, 1>(buf, tw, l2len, nrows, rs, Ns, lns);
       Ns <<= 1; lns += 1; rem -= 1;
     }
   }
 }
 
 template <typename T>
-__device__ void fill_twiddles(cx<T>* tw, int l2len) {
-  const int len = 1 << l2len;
-  for (int i = threadIdx.x; i < len; i += kThreads) tw[i] = expmi<T>(2 * (int64_t)i, len);
-  __syncthreads();
+__device__ void fill_table(cx<T>* tab, int n, int64_t den2) {
+  // tab[i] = exp(-2 pi i i / den2) for i < n (den2 a power of two)
+  const T inv = T(1) / (T)den2;
+  for (int i = threadIdx.x; i < n; i += kThreads) tab[i] = expmi2<T>(2 * (int64_t)i, inv);
 }
 
 // Segment: `rows` rows sharing one length n and one plan.
@@ -174,6 +205,7 @@ struct Seg {
   int32_t rows, n, l1, l2, cw, flags;
   int64_t base1, base2, base3;  // first global item of K1, K2, K3
   int32_t per1, per2;           // items per row for K1/K3, K2
+  int32_t r2, pad2;             // k1 values per K2 item
 };
 
 enum Mode { kFilter = 0, kFSpec = 1, kBlue = 2, kBSpec = 3 };
@@ -189,6 +221,8 @@ __device__ inline int find_seg(const Seg* segs, int nseg, int64_t g, int which) 
 }
 
 // ---- K1: column FFTs -------------------------------------------------------
+// LDS: cw padded rows of length P1 (one per matrix column j2), then the P1
+// twiddle table and the P2 table exp(-2 pi i j/P) for the four-step twiddle.
 template <typename T, typename TIn, int MODE>
 __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ segs, int nseg,
                                                       const TIn* __restrict__ in, cx<T>* __restrict__ tbuf) {
@@ -198,38 +232,59 @@ __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ se
   const int64_t local = (int64_t)blockIdx.x - sg.base1;
   const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
   const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
+  const int lcw = 31 - __clz(cw);
+  const int rs = row_stride(P1);
   const int64_t P = (int64_t)P1 * P2;
   const int64_t m = sg.n - 1, N = 2 * m;
-  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [P1][cw]
-  cx<T>* tw = buf + P1 * cw;                          // [P1]
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [cw][rs]
+  cx<T>* tw = buf + cw * rs;                          // [P1]
+  cx<T>* tb2 = tw + P1;                               // [P2]
   const TIn* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * cw;
+  fill_table<T>(tw, P1, P1);
+  fill_table<T>(tb2, P2, P);
+  constexpr int UNR = 8;  // loads in flight per thread
   if (MODE == kFilter || MODE == kFSpec) {
     // real input packed two per complex point: a_(2j), a_(2j+1)
-    T* rb = reinterpret_cast<T*>(buf);
     const int64_t L = 2 * P;
-    for (int idx = threadIdx.x; idx < P1 * cw * 2; idx += kThreads) {
-      int j1 = idx / (2 * cw), w = idx - j1 * 2 * cw;
-      int64_t i = 2 * ((int64_t)P2 * j1 + j2_0) + w;  // a-index
-      T val = T(0);
-      if (MODE == kFilter) {
-        if (i <= 3 * m) {
-          int64_t t = i - m;
-          int64_t src = t < 0 ? -t : (t <= m ? t : 2 * m - t);
-          val = (T)x[src];
+    const int total = P1 * cw * 2;
+    for (int base = threadIdx.x; base < total; base += kThreads * UNR) {
+      T val[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        val[u] = T(0);
+        if (idx < total) {
+          int j1 = idx >> (lcw + 1), w = idx & (2 * cw - 1);
+          int64_t i = 2 * ((int64_t)P2 * j1 + j2_0) + w;  // a-index
+          if (MODE == kFilter) {
+            if (i <= 3 * m) {
+              int64_t t = i - m;
+              int64_t src = t < 0 ? -t : (t <= m ? t : 2 * m - t);
+              val[u] = (T)x[src];
+            }
+          } else {
+            // symmetric embedding of h (rows of DCT-I(W)): b_i = h_i (i < m),
+            // h_m/2 at i = m and L-m, h_(L-i) for L-m < i < L
+            if (i < m) val[u] = (T)x[i];
+            else if (i == m || i == L - m) val[u] = (T)x[m] * T(0.5);
+            else if (i > L - m) val[u] = (T)x[L - i];
+          }
         }
-      } else {
-        // symmetric embedding of h (rows of DCT-I(W)): b_i = h_i (i < m),
-        // h_m/2 at i = m and L-m, h_(L-i) for L-m < i < L
-        if (i < m) val = (T)x[i];
-        else if (i == m || i == L - m) val = (T)x[m] * T(0.5);
-        else if (i > L - m) val = (T)x[L - i];
       }
-      rb[2 * (j1 * cw) + w] = val;  // buf[j1][w/2].(re|im)
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        if (idx < total) {
+          int j1 = idx >> (lcw + 1), w = idx & (2 * cw - 1);
+          T* dst = reinterpret_cast<T*>(buf + (w >> 1) * rs + pad8(j1));
+          dst[w & 1] = val[u];
+        }
+      }
     }
   } else {
     for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
-      int j1 = idx / cw, c = idx - j1 * cw;
+      int j1 = idx >> lcw, c = idx & (cw - 1);
       int64_t j = (int64_t)P2 * j1 + j2_0 + c;
       cx<T> val{T(0), T(0)};
       if (MODE == kBlue) {
@@ -242,116 +297,158 @@ __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ se
         int64_t t = j <= m ? j : (j >= P - m ? P - j : -1);
         if (t >= 0) val = cconj(expmi<T>((t * t) % (2 * N), N));
       }
-      buf[j1 * cw + c] = val;
+      buf[c * rs + pad8(j1)] = val;
     }
   }
-  fill_twiddles<T>(tw, l1);  // has a barrier
-  lds_fft<T, false>(buf, tw, l1, cw, 1, cw);
+  __syncthreads();
+  lds_fft<T, false>(buf, tw, l1, cw, rs);
   cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
-    int k1 = idx / cw, c = idx - k1 * cw;
+    int k1 = idx >> lcw, c = idx & (cw - 1);
     int64_t j2 = j2_0 + c;
-    cx<T> v = cmul(buf[k1 * cw + c], expmi<T>(2 * ((j2 * k1) & (P - 1)), P));
+    int64_t a = (j2 * k1) & (P - 1);  // w_P^a = w_P1^(a / P2) * w_P^(a % P2)
+    cx<T> v = cmul(buf[c * rs + pad8(k1)], cmul(tw[a >> l2], tb2[a & (P2 - 1)]));
     trow[(int64_t)k1 * P2 + j2] = v;
   }
 }
 
 // ---- K2: row FFTs + spectral work + inverse row FFTs ----------------------
-// Real modes process row pair (k1, P1-k1); complex modes one row.
-template <typename T, int MODE>
+// One workgroup takes r2 consecutive k1 values of one row (TOD detector
+// mode).  Real modes also load each partner row P1-k1, where the real-FFT
+// unpacking partner C_(P-k) lives, so every element's partner is in LDS and
+// one thread handles the pair (k, P-k) end to end: unpack, weight, repack,
+// in place (w_L^(P-k) = -conj(w_L^k), so one sincospi per pair).
+// TS: storage type of real spectra (the FP32 filter keeps FP32 spectra that
+// were computed in FP64).
+template <typename T, typename TS, int MODE>
 __global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ segs, int nseg,
-                                                      cx<T>* __restrict__ tbuf, T* __restrict__ spec_r,
-                                                      cx<T>* __restrict__ spec_c) {
+                                                      cx<T>* __restrict__ tbuf) {
   extern __shared__ unsigned char smem_raw[];
+  __shared__ int slot_k1[16], slot_partner[16];
+  __shared__ int nslot_sh;
   const int s = find_seg(segs, nseg, blockIdx.x, 2);
   const Seg sg = segs[s];
   const int64_t local = (int64_t)blockIdx.x - sg.base2;
-  const int row = (int)(local / sg.per2), k1 = (int)(local % sg.per2);
+  const int row = (int)(local / sg.per2), grp = (int)(local % sg.per2);
   const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2;
+  const int rs = row_stride(P2);
   const int64_t P = (int64_t)P1 * P2;
-  cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
-  cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);  // [2][P2]
-  cx<T>* Yb = X + 2 * P2;                          // [2][P2] (real modes)
-  cx<T>* tw = Yb + 2 * P2;                         // [P2]
   const bool real_mode = (MODE == kFilter || MODE == kFSpec);
-  const int k1b = real_mode ? ((P1 - k1) & (P1 - 1)) : k1;
-  const bool pair = real_mode && (k1b != k1);
-  const int nrow = pair ? 2 : 1;
-  for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
-    int r = idx >> l2, k2 = idx & (P2 - 1);
-    int kk1 = r ? k1b : k1;
-    X[idx] = trow[(int64_t)kk1 * P2 + k2];
+  if (threadIdx.x == 0) {
+    int ns = 0;
+    const int kmax = real_mode ? P1 / 2 : P1 - 1;
+    for (int q = 0; q < sg.r2; q++) {
+      int k1 = grp * sg.r2 + q;
+      if (k1 > kmax) break;
+      int a = ns++;
+      slot_k1[a] = k1;
+      slot_partner[a] = a;
+      int k1b = (P1 - k1) & (P1 - 1);
+      if (real_mode && k1b != k1) {
+        int b = ns++;
+        slot_k1[b] = k1b;
+        slot_partner[a] = b;
+        slot_partner[b] = a;
+      }
+    }
+    nslot_sh = ns;
   }
-  fill_twiddles<T>(tw, l2);
-  lds_fft<T, false>(X, tw, l2, nrow, P2, 1);
+  cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);             // [nslot][rs]
+  cx<T>* tw = X + (real_mode ? 2 * sg.r2 : sg.r2) * rs;      // [P2]  exp(-2 pi i j/P2)
+  cx<T>* tb1 = tw + P2;                                       // [P1]  exp(-2 pi i j/P1)
+  cx<T>* tb2 = tb1 + P1;                                      // [P2]  exp(-2 pi i j/P)
+  fill_table<T>(tw, P2, P2);
+  fill_table<T>(tb1, P1, P1);
+  fill_table<T>(tb2, P2, P);
+  __syncthreads();
+  const int nslot = nslot_sh;
+  cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  {
+    constexpr int UNR = 8;
+    const int total = nslot * P2;
+    for (int base = threadIdx.x; base < total; base += kThreads * UNR) {
+      cx<T> v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        if (idx < total) v[u] = trow[(int64_t)slot_k1[idx >> l2] * P2 + (idx & (P2 - 1))];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        if (idx < total) X[(idx >> l2) * rs + pad8(idx & (P2 - 1))] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  lds_fft<T, false>(X, tw, l2, nslot, rs);
+  auto at = [&](int idx) -> cx<T>& { return X[(idx >> l2) * rs + pad8(idx & (P2 - 1))]; };
   if (MODE == kBSpec) {
     cx<T>* srow = reinterpret_cast<cx<T>*>(sg.s_ptr);
-    for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) srow[(int64_t)k1 * P2 + k2] = X[k2];
+    for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads)
+      srow[(int64_t)slot_k1[idx >> l2] * P2 + (idx & (P2 - 1))] = at(idx);
     return;
   }
   if (MODE == kBlue) {
     const cx<T>* srow = reinterpret_cast<const cx<T>*>(sg.s_ptr);
-    for (int k2 = threadIdx.x; k2 < P2; k2 += kThreads) X[k2] = cmul(X[k2], srow[(int64_t)k1 * P2 + k2]);
+    for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads)
+      at(idx) = cmul(at(idx), srow[(int64_t)slot_k1[idx >> l2] * P2 + (idx & (P2 - 1))]);
     __syncthreads();
   } else {
-    // unpack the real spectrum A_k = E_k + w_L^k O_k for k = kk1 + P1*k2
-    const int64_t L = 2 * P;
-    T* srow = reinterpret_cast<T*>(sg.s_ptr) + (int64_t)row * sg.s_stride;
-    T YPval = T(0);
-    for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
-      int r = idx >> l2, k2 = idx & (P2 - 1);
-      int kk1 = r ? k1b : k1;
-      // partner index P-k
-      int pr, pk2;
-      if (kk1 == 0) { pr = 0; pk2 = (P2 - k2) & (P2 - 1); }
-      else { pr = (kk1 == k1) ? (pair ? 1 : 0) : 0; pk2 = P2 - 1 - k2; }
-      cx<T> C = X[idx], Cp = cconj(X[pr * P2 + pk2]);
-      cx<T> E = cscale(cadd(C, Cp), T(0.5));
-      cx<T> D = csub(C, Cp);
-      cx<T> O = {D.im * T(0.5), -D.re * T(0.5)};  // (C - conj Cp)/(2i)
+    TS* srow = reinterpret_cast<TS*>(sg.s_ptr) + (int64_t)row * sg.s_stride;
+    const T invP = T(1) / (T)P;
+    for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads) {
+      int sl = idx >> l2, k2 = idx & (P2 - 1);
+      int kk1 = slot_k1[sl];
+      int pidx = (kk1 == 0) ? (sl * P2 + ((P2 - k2) & (P2 - 1))) : (slot_partner[sl] * P2 + (P2 - 1 - k2));
+      if (pidx < idx) continue;  // the pair is handled by its lower index
       int64_t k = kk1 + (int64_t)P1 * k2;
-      cx<T> A = cadd(E, cmul(expmi<T>(k, P), O));  // w_L^k = exp(-i pi k/P)
+      cx<T> C = at(idx), Cq = at(pidx);
+      // E_k = (C_k + conj C_(P-k))/2, O_k = (C_k - conj C_(P-k))/(2i); the
+      // partner's E, O are conj(E_k), conj(O_k).
+      cx<T> E = cscale(cadd(C, cconj(Cq)), T(0.5));
+      cx<T> Dd = csub(C, cconj(Cq));
+      cx<T> O = {Dd.im * T(0.5), -Dd.re * T(0.5)};
+      cx<T> w = expmi2<T>(k, invP);  // w_L^k; w_L^(P-k) = -conj(w)
+      cx<T> A = cadd(E, cmul(w, O));
+      cx<T> Aq = csub(cconj(E), cmul(cconj(w), cconj(O)));
+      int pk1 = slot_k1[pidx >> l2], pk2 = pidx & (P2 - 1);
       if (MODE == kFSpec) {
-        srow[(int64_t)kk1 * P2 + k2] = A.re;
-        if (k == 0) srow[P] = E.re - O.re;  // A_P
-      } else {
-        T b = srow[(int64_t)kk1 * P2 + k2];
-        Yb[idx] = cscale(A, b);
-        if (k == 0) YPval = (E.re - O.re) * srow[P];
+        srow[(int64_t)kk1 * P2 + k2] = (TS)A.re;
+        if (pidx != idx) srow[(int64_t)pk1 * P2 + pk2] = (TS)Aq.re;  // self pair k = 0: Aq is A_P
+        if (k == 0) srow[P] = (TS)(E.re - O.re);                       // A_P
+        continue;
       }
-      (void)L;
+      T b = (T)srow[(int64_t)kk1 * P2 + k2];
+      T bq = (T)srow[(int64_t)pk1 * P2 + pk2];
+      cx<T> Y = cscale(A, b), Yq = cscale(Aq, bq);
+      if (k == 0) {
+        // self pair: the inverse partner of Y_0 is Y_P (real)
+        T yP = (E.re - O.re) * (T)srow[P];
+        cx<T> Ep = {T(0.5) * (Y.re + yP), T(0.5) * Y.im};
+        cx<T> Op = {T(0.5) * (Y.re - yP), T(0.5) * Y.im};
+        at(idx) = {Ep.re - Op.im, Ep.im + Op.re};
+        continue;
+      }
+      // D_k = E'_k + i O'_k, E'_k = (Y_k + conj Y_(P-k))/2,
+      // O'_k = (Y_k - conj Y_(P-k)) conj(w_L^k)/2; same for P-k with w -> -conj(w)
+      cx<T> Ep = cscale(cadd(Y, cconj(Yq)), T(0.5));
+      cx<T> Op = cmul(cscale(csub(Y, cconj(Yq)), T(0.5)), cconj(w));
+      cx<T> Eq = cscale(cadd(Yq, cconj(Y)), T(0.5));
+      cx<T> Oq = cmul(cscale(csub(Yq, cconj(Y)), T(0.5)), cx<T>{-w.re, -w.im});
+      at(idx) = {Ep.re - Op.im, Ep.im + Op.re};
+      if (pidx != idx) at(pidx) = {Eq.re - Oq.im, Eq.im + Oq.re};
     }
     if (MODE == kFSpec) return;
-    // Y_P is produced by the thread that owns k = 0; share it via LDS
-    __shared__ T yp_sh;
-    if (k1 == 0 && threadIdx.x == 0) yp_sh = YPval;
-    __syncthreads();
-    // pack for the inverse: D_k = E'_k + i O'_k
-    for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
-      int r = idx >> l2, k2 = idx & (P2 - 1);
-      int kk1 = r ? k1b : k1;
-      int64_t k = kk1 + (int64_t)P1 * k2;
-      cx<T> Y = Yb[idx], Ym;
-      if (k == 0) {
-        Ym = {yp_sh, T(0)};
-      } else {
-        int pr, pk2;
-        if (kk1 == 0) { pr = 0; pk2 = (P2 - k2) & (P2 - 1); }
-        else { pr = (kk1 == k1) ? (pair ? 1 : 0) : 0; pk2 = P2 - 1 - k2; }
-        Ym = cconj(Yb[pr * P2 + pk2]);
-      }
-      cx<T> Ep = cscale(cadd(Y, Ym), T(0.5));
-      cx<T> Op = cmul(cscale(csub(Y, Ym), T(0.5)), cconj(expmi<T>(k, P)));
-      X[idx] = {Ep.re - Op.im, Ep.im + Op.re};  // Ep + i*Op
-    }
     __syncthreads();
   }
-  lds_fft<T, true>(X, tw, l2, nrow, P2, 1);
-  for (int idx = threadIdx.x; idx < nrow * P2; idx += kThreads) {
-    int r = idx >> l2, j2 = idx & (P2 - 1);
-    int kk1 = r ? k1b : k1;
-    cx<T> v = cmul(X[idx], cconj(expmi<T>(2 * (((int64_t)j2 * kk1) & (P - 1)), P)));
-    trow[(int64_t)kk1 * P2 + j2] = v;
+  lds_fft<T, true>(X, tw, l2, nslot, rs);
+  for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads) {
+    int sl = idx >> l2, j2 = idx & (P2 - 1);
+    int kk1 = slot_k1[sl];
+    int64_t a = ((int64_t)j2 * kk1) & (P - 1);  // conj(w_P^a) from two table entries
+    cx<T> tw_a = cmul(tb1[a >> l2], tb2[a & (P2 - 1)]);
+    trow[(int64_t)kk1 * P2 + j2] = cmul(at(idx), cconj(tw_a));
   }
 }
 
@@ -365,39 +462,61 @@ __global__ void __launch_bounds__(kThreads) k3_kernel(const Seg* __restrict__ se
   const int64_t local = (int64_t)blockIdx.x - sg.base3;
   const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
   const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
+  const int lcw = 31 - __clz(cw);
+  const int rs = row_stride(P1);
   const int64_t P = (int64_t)P1 * P2;
   const int64_t m = sg.n - 1, N = 2 * m;
-  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* tw = buf + P1 * cw;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [cw][rs]
+  cx<T>* tw = buf + cw * rs;                          // [P1]
   const cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   const int j2_0 = jb * cw;
-  for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
-    int k1 = idx / cw, c = idx - k1 * cw;
-    buf[k1 * cw + c] = trow[(int64_t)k1 * P2 + j2_0 + c];
+  fill_table<T>(tw, P1, P1);
+  {
+    constexpr int UNR = 8;
+    const int total = P1 * cw;
+    for (int base = threadIdx.x; base < total; base += kThreads * UNR) {
+      cx<T> v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        if (idx < total) {
+          int k1 = idx >> lcw, c = idx & (cw - 1);
+          v[u] = trow[(int64_t)k1 * P2 + j2_0 + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        int idx = base + u * kThreads;
+        if (idx < total) {
+          int k1 = idx >> lcw, c = idx & (cw - 1);
+          buf[c * rs + pad8(k1)] = v[u];
+        }
+      }
+    }
   }
-  fill_twiddles<T>(tw, l1);
-  lds_fft<T, true>(buf, tw, l1, cw, 1, cw);
+  __syncthreads();
+  lds_fft<T, true>(buf, tw, l1, cw, rs);
   const T scale = T(1) / (T)P;
   TOut* y = out + sg.out_off + (int64_t)row * sg.out_stride;
   if (MODE == kFilter) {
-    const T* rb = reinterpret_cast<const T*>(buf);
     const bool halve = sg.flags & 1;
     for (int idx = threadIdx.x; idx < P1 * cw * 2; idx += kThreads) {
-      int j1 = idx / (2 * cw), w = idx - j1 * 2 * cw;
+      int j1 = idx >> (lcw + 1), w = idx & (2 * cw - 1);
       int64_t i = 2 * ((int64_t)P2 * j1 + j2_0) + w;
       int64_t k = i - m;
       if (k >= 0 && k <= m) {
-        T v = rb[2 * (j1 * cw) + w] * scale;
+        const T* src = reinterpret_cast<const T*>(buf + (w >> 1) * rs + pad8(j1));
+        T v = src[w & 1] * scale;
         if (halve && (k == 0 || k == m)) v *= T(0.5);
         y[k] = (TOut)v;
       }
     }
   } else {  // kBlue
     for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
-      int j1 = idx / cw, c = idx - j1 * cw;
+      int j1 = idx >> lcw, c = idx & (cw - 1);
       int64_t k = (int64_t)P2 * j1 + j2_0 + c;
       if (k <= m) {
-        cx<T> w = cscale(buf[j1 * cw + c], scale);
+        cx<T> w = cscale(buf[c * rs + pad8(j1)], scale);
         cx<T> ch = expmi<T>((k * k) % (2 * N), N);
         y[k] = (TOut)(ch.re * w.re - ch.im * w.im);
       }
@@ -427,10 +546,19 @@ Geometry geometry(int64_t P_min, int elem_bytes) {
   g.l2 = q - g.l1;
   g.P = int64_t(1) << q;
   int P1 = 1 << g.l1, P2 = 1 << g.l2;
-  int cw = std::min(16, std::max(1, kMaxWG / P1));
+  // columns per K1/K3 workgroup: MK_FFT_CW (a power of two, set by the
+  // autotuner) caps the default of 16
+  static int cw_cap = [] {
+    const char* e = getenv("MK_FFT_CW");
+    int v = e ? atoi(e) : 16;
+    int c = 1;
+    while (c * 2 <= v && c < 64) c *= 2;
+    return c;
+  }();
+  int cw = std::min(cw_cap, std::max(1, kMaxWG / P1));
   cw = std::min(cw, P2);
-  // keep (cw + 1) * P1 complex values within the LDS budget
-  while (cw > 1 && (int64_t)(cw + 1) * P1 * 2 * elem_bytes > kMaxLds) cw >>= 1;
+  // keep the K1/K3 tile plus twiddle tables within the LDS budget
+  while (cw > 1 && ((int64_t)cw * row_stride(P1) + P1 + P2) * 2 * elem_bytes > kMaxLds) cw >>= 1;
   g.cw = cw;
   return g;
 }
@@ -453,14 +581,15 @@ void allow_lds_all() {
   allow_big_lds(k1_kernel<double, double, kBlue>);
   allow_big_lds(k1_kernel<float, float, kBSpec>);
   allow_big_lds(k1_kernel<double, double, kBSpec>);
-  allow_big_lds(k2_kernel<float, kFilter>);
-  allow_big_lds(k2_kernel<double, kFilter>);
-  allow_big_lds(k2_kernel<float, kFSpec>);
-  allow_big_lds(k2_kernel<double, kFSpec>);
-  allow_big_lds(k2_kernel<float, kBlue>);
-  allow_big_lds(k2_kernel<double, kBlue>);
-  allow_big_lds(k2_kernel<float, kBSpec>);
-  allow_big_lds(k2_kernel<double, kBSpec>);
+  allow_big_lds(k2_kernel<float, float, kFilter>);
+  allow_big_lds(k2_kernel<double, double, kFilter>);
+  allow_big_lds(k2_kernel<float, float, kFSpec>);
+  allow_big_lds(k2_kernel<double, double, kFSpec>);
+  allow_big_lds(k2_kernel<double, float, kFSpec>);
+  allow_big_lds(k2_kernel<float, float, kBlue>);
+  allow_big_lds(k2_kernel<double, double, kBlue>);
+  allow_big_lds(k2_kernel<float, float, kBSpec>);
+  allow_big_lds(k2_kernel<double, double, kBSpec>);
   allow_big_lds(k3_kernel<float, float, kFilter>);
   allow_big_lds(k3_kernel<double, double, kFilter>);
   allow_big_lds(k3_kernel<float, float, kBlue>);
@@ -468,8 +597,22 @@ void allow_lds_all() {
   (void)hipGetLastError();  // an unsupported attribute must not poison later launch checks
 }
 
-size_t lds_k13(const Geometry& g, int eb) { return (size_t)((1 << g.l1) * g.cw + (1 << g.l1)) * 2 * eb; }
-size_t lds_k2(const Geometry& g, int eb) { return (size_t)(5 * (1 << g.l2)) * 2 * eb + 64; }
+size_t lds_k13(const Geometry& g, int eb) {
+  return (size_t)(g.cw * row_stride(1 << g.l1) + (1 << g.l1) + (1 << g.l2)) * 2 * eb;
+}
+// k1 values per K2 workgroup: slots (2 per k1 for real modes) * P2 <= kMaxWG
+int k2_r2(const Geometry& g, int eb, bool real) {
+  int P2 = 1 << g.l2, P1 = 1 << g.l1;
+  int r = 4;
+  while (r > 1 && ((real ? 2 : 1) * r * P2 > kMaxWG ||
+                   (size_t)((real ? 2 * r : r) * row_stride(P2) + 2 * P2 + P1) * 2 * eb > (size_t)kMaxLds))
+    r >>= 1;
+  return std::min(r, real ? P1 / 2 + 1 : P1);
+}
+size_t lds_k2(const Geometry& g, int eb, bool real = true) {
+  int r = k2_r2(g, eb, real);
+  return (size_t)((real ? 2 * r : r) * row_stride(1 << g.l2) + 2 * (1 << g.l2) + (1 << g.l1)) * 2 * eb + 64;
+}
 
 }  // namespace
 
@@ -491,6 +634,19 @@ struct mk_filt_plan {
 };
 
 namespace {
+// The work buffer (rows * P complex) is allocated on first use and can be
+// released between uses; spectra stay resident.
+int ensure_tbuf(const mk_filt_plan* cp) {
+  mk_filt_plan* p = const_cast<mk_filt_plan*>(cp);
+  if (p->tbuf) return MK_OK;
+  if (hipMalloc(&p->tbuf, std::max<int64_t>(1, p->t_elems) * 2 * p->elem_bytes) != hipSuccess) {
+    p->tbuf = nullptr;
+    mk::set_error("filter plan: work buffer allocation (%lld B) failed", (long long)(p->t_elems * 2 * p->elem_bytes));
+    return MK_ERR_ALLOC;
+  }
+  return MK_OK;
+}
+
 int upload(mk_filt_plan* p) {
   if (p->nseg && hipMemcpy(p->d_segs, p->h_segs.data(), p->nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
     mk::set_error("filter plan: descriptor upload failed");
@@ -528,14 +684,15 @@ int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<S
     s.t_off = toff; s.t_stride = g.P;
     s.s_off = soff; s.s_stride = filter ? g.P + 1 : 0;
     s.per1 = (int32_t)((int64_t(1) << g.l2) / g.cw);
-    s.per2 = filter ? (1 << (g.l1 - 1)) + 1 : (1 << g.l1);
+    s.r2 = k2_r2(g, 8, filter);  // same grouping for FP32 and FP64 passes
+    s.per2 = (int32_t)mk::cdiv(filter ? (1 << (g.l1 - 1)) + 1 : (1 << g.l1), s.r2);
     s.base1 = b1; s.base2 = b2; s.base3 = b1;
     b1 += rows * s.per1;
     b2 += rows * s.per2;
     toff += rows * g.P;
     soff += filter ? rows * (g.P + 1) : g.P;
     l1max = std::max(l1max, lds_k13(g, eb));
-    l2max = std::max(l2max, lds_k2(g, eb));
+    l2max = std::max(l2max, lds_k2(g, eb, filter));
     MK_REQUIRE(l1max <= (size_t)kMaxLds && l2max <= (size_t)kMaxLds, MK_ERR_ARG,
                "segment %d: n=%lld needs more LDS than the %d-byte budget at %d-byte precision", i,
                (long long)n, kMaxLds, eb);
@@ -545,19 +702,19 @@ int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<S
   return MK_OK;
 }
 
-template <typename T>
-void launch_fspec(mk_filt_plan* p, const void* wdct, hipStream_t st) {
-  k1_kernel<T, double, kFSpec><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg, (const double*)wdct,
-                                                                           (cx<T>*)p->tbuf);
-  k2_kernel<T, kFSpec><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf,
-                                                                   (T*)p->spec, nullptr);
+// Spectrum pass in arithmetic type T into spectra of type TS; tb holds
+// rows*P complex<T>; lds1/lds2 sized for T.
+template <typename T, typename TS>
+void launch_fspec(mk_filt_plan* p, const void* wdct, void* tb, size_t lds1, size_t lds2, hipStream_t st) {
+  k1_kernel<T, double, kFSpec><<<(unsigned)p->n1, kThreads, lds1, st>>>(p->d_segs, p->nseg, (const double*)wdct,
+                                                                        (cx<T>*)tb);
+  k2_kernel<T, TS, kFSpec><<<(unsigned)p->n2, kThreads, lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)tb);
 }
 
 template <typename T, typename TIO>
 void launch_filter(const mk_filt_plan* p, const TIO* in, TIO* out, hipStream_t st) {
   k1_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg, in, (cx<T>*)p->tbuf);
-  k2_kernel<T, kFilter><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf,
-                                                                    (T*)p->spec, nullptr);
+  k2_kernel<T, T, kFilter><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf);
   k3_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg,
                                                                          (const cx<T>*)p->tbuf, out);
 }
@@ -579,7 +736,6 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
   if (st) { free_filt(p); return st; }
   MK_REQUIRE(p->n1 < INT32_MAX && p->n2 < INT32_MAX, (free_filt(p), MK_ERR_ARG), "too many work items");
   if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
-      hipMalloc(&p->tbuf, std::max<int64_t>(1, p->t_elems) * 2 * elem_bytes) != hipSuccess ||
       hipMalloc(&p->spec, std::max<int64_t>(1, p->s_elems) * elem_bytes) != hipSuccess) {
     free_filt(p);
     mk::set_error("filter plan: device allocation failed");
@@ -648,10 +804,28 @@ extern "C" int mk_filt_plan_set_weights(mk_filt_plan* p, const double* wdct, voi
     MK_REQUIRE(!p->borrowed[i], MK_ERR_ARG, "plan borrows spectra; set_weights would overwrite the source plan");
   if (p->n1 == 0) { p->have_spec = true; return MK_OK; }
   hipStream_t st = mk::as_stream(stream);
-  if (p->elem_bytes == 4) launch_fspec<float>(p, wdct, st);
-  else launch_fspec<double>(p, wdct, st);
+  // The spectrum is always computed in FP64 (a transient FP64 work buffer)
+  // and stored in the plan's precision, unless FP64 LDS tiles would not fit.
+  size_t l1_64 = 0, l2_64 = 0;
+  for (const Seg& sg : p->h_segs) {
+    Geometry g{sg.l1, sg.l2, sg.cw, int64_t(1) << (sg.l1 + sg.l2)};
+    l1_64 = std::max(l1_64, lds_k13(g, 8));
+    l2_64 = std::max(l2_64, lds_k2(g, 8));
+  }
+  const bool fp64_pass = l1_64 <= (size_t)kMaxLds && l2_64 <= (size_t)kMaxLds;
+  void* tb = nullptr;
+  if (fp64_pass) {
+    MK_HIP(hipMalloc(&tb, std::max<int64_t>(1, p->t_elems) * 16));
+    if (p->elem_bytes == 4) launch_fspec<double, float>(p, wdct, tb, l1_64, l2_64, st);
+    else launch_fspec<double, double>(p, wdct, tb, l1_64, l2_64, st);
+  } else {
+    MK_HIP(hipMalloc(&tb, std::max<int64_t>(1, p->t_elems) * 8));
+    launch_fspec<float, float>(p, wdct, tb, p->lds1, p->lds2, st);
+  }
   int r = mk::check_launch("filter spectrum");
   if (r == MK_OK) p->have_spec = true;
+  MK_HIP(hipStreamSynchronize(st));
+  MK_HIP(hipFree(tb));
   return r;
 }
 
@@ -660,6 +834,8 @@ extern "C" int mk_filt_plan_exec_f32(const mk_filt_plan* p, const float* in, flo
   MK_REQUIRE(p->have_spec, MK_ERR_ARG, "filter weights not set");
   MK_REQUIRE(p->elem_bytes == 4, MK_ERR_ARG, "plan was created for FP64");
   if (p->n1 == 0) return MK_OK;
+  int r = ensure_tbuf(p);
+  if (r) return r;
   launch_filter<float, float>(p, in, out, mk::as_stream(stream));
   return mk::check_launch("filter exec f32");
 }
@@ -669,8 +845,20 @@ extern "C" int mk_filt_plan_exec_f64(const mk_filt_plan* p, const double* in, do
   MK_REQUIRE(p->have_spec, MK_ERR_ARG, "filter weights not set");
   MK_REQUIRE(p->elem_bytes == 8, MK_ERR_ARG, "plan was created for FP32");
   if (p->n1 == 0) return MK_OK;
+  int r = ensure_tbuf(p);
+  if (r) return r;
   launch_filter<double, double>(p, in, out, mk::as_stream(stream));
   return mk::check_launch("filter exec f64");
+}
+
+extern "C" int mk_filt_plan_release_work(mk_filt_plan* p, void* stream) {
+  MK_REQUIRE(p, MK_ERR_ARG, "null plan");
+  if (p->tbuf) {
+    MK_HIP(hipStreamSynchronize(mk::as_stream(stream)));
+    MK_HIP(hipFree(p->tbuf));
+    p->tbuf = nullptr;
+  }
+  return MK_OK;
 }
 
 extern "C" int mk_filt_plan_set_halve(mk_filt_plan* p, int halve_edges) {
@@ -724,9 +912,9 @@ int dct1_impl(const TIO* x, TIO* y, int64_t nrow, int64_t n, void* work, int64_t
   MK_HIP(hipMemcpyAsync(d_sv, &sv, sizeof(Seg), hipMemcpyHostToDevice, st));
   MK_HIP(hipStreamSynchronize(st));
   k1_kernel<T, TIO, kBSpec><<<(unsigned)s0.per1, kThreads, l1, st>>>(d_sv, 1, x, V);
-  k2_kernel<T, kBSpec><<<(unsigned)s0.per2, kThreads, l2, st>>>(d_sv, 1, V, nullptr, V);
+  k2_kernel<T, T, kBSpec><<<(unsigned)s0.per2, kThreads, l2, st>>>(d_sv, 1, V);
   k1_kernel<T, TIO, kBlue><<<(unsigned)n1, kThreads, l1, st>>>(d_seg, 1, x, tb);
-  k2_kernel<T, kBlue><<<(unsigned)n2, kThreads, l2, st>>>(d_seg, 1, tb, nullptr, V);
+  k2_kernel<T, T, kBlue><<<(unsigned)n2, kThreads, l2, st>>>(d_seg, 1, tb);
   k3_kernel<T, TIO, kBlue><<<(unsigned)n1, kThreads, l1, st>>>(d_seg, 1, tb, y);
   return mk::check_launch("dct1");
 }

@@ -142,10 +142,17 @@ __global__ void __launch_bounds__(256) t2m_keys_kernel(const int32_t* __restrict
   }
 }
 
+// Run lengths of the sorted tile keys without atomics: the last sample of
+// each run writes end = i+1 and the first writes start = i (count = end -
+// start; tiles without samples keep 0/0).
 __global__ void __launch_bounds__(256) t2m_count_kernel(const uint32_t* __restrict__ key, int64_t n,
-                                                        int32_t* __restrict__ count) {
+                                                        int32_t* __restrict__ start, int32_t* __restrict__ end) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) atomicAdd(count + key[i], 1);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t k = key[i];
+    if (i == 0 || key[i - 1] != k) start[k] = (int32_t)i;
+    if (i == n - 1 || key[i + 1] != k) end[k] = (int32_t)(i + 1);
+  }
 }
 
 // Writes local pixel per sorted entry.
@@ -323,7 +330,7 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   std::vector<T2MItem> h_items;
   const int64_t segmax = std::min<int64_t>(kSegment, std::max<int64_t>(nsamp, 1));
   DevBuf keys, keys2, idx, cnt, nbad, tmp;
-  if (keys.alloc(segmax * 4) || keys2.alloc(segmax * 4) || idx.alloc(segmax * 4) || cnt.alloc(ntiles * 4 + 4) ||
+  if (keys.alloc(segmax * 4) || keys2.alloc(segmax * 4) || idx.alloc(segmax * 4) || cnt.alloc(ntiles * 8 + 8) ||
       nbad.alloc(8))
     return fail(MK_ERR_ALLOC);
   size_t tmp_bytes = 0;
@@ -334,10 +341,10 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   if (tmp.alloc(tmp_bytes)) return fail(MK_ERR_ALLOC);
   int end_bit = 1;
   while ((int64_t(1) << end_bit) < ntiles) end_bit++;
-  std::vector<int32_t> h_cnt(ntiles);
+  std::vector<int32_t> h_cnt(2 * ntiles);
   for (int64_t s0 = 0; s0 < nsamp; s0 += kSegment) {
     const int64_t n = std::min<int64_t>(kSegment, nsamp - s0);
-    if (hipMemsetAsync(nbad.p, 0, 8, stream) != hipSuccess || hipMemsetAsync(cnt.p, 0, ntiles * 4, stream) != hipSuccess)
+    if (hipMemsetAsync(nbad.p, 0, 8, stream) != hipSuccess || hipMemsetAsync(cnt.p, 0, ntiles * 8, stream) != hipSuccess)
       return fail((mk::set_error("memset failed"), MK_ERR_HIP));
     t2m_keys_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix + s0, n, nx, ny, tile_x, tile_y, pl->ntile_y,
                                                                (uint32_t*)keys.p, (uint32_t*)idx.p, (int64_t*)nbad.p);
@@ -354,16 +361,17 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
     if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, (uint32_t*)keys.p, (uint32_t*)keys2.p, (uint32_t*)idx.p,
                                            (uint32_t*)(pl->entry + s0), (int)n, 0, end_bit, stream) != hipSuccess)
       return fail((mk::set_error("hipcub sort failed"), MK_ERR_HIP));
-    t2m_count_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>((uint32_t*)keys2.p, n, (int32_t*)cnt.p);
+    t2m_count_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>((uint32_t*)keys2.p, n, (int32_t*)cnt.p,
+                                                                (int32_t*)cnt.p + ntiles);
     t2m_lpix_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix, s0, pl->entry + s0, n, ny, tile_x, tile_y,
                                                                pl->lpix + s0);
     if (mk::check_launch("t2m_lpix_kernel")) return fail(MK_ERR_HIP);
-    if (hipMemcpyAsync(h_cnt.data(), cnt.p, ntiles * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+    if (hipMemcpyAsync(h_cnt.data(), cnt.p, ntiles * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
       return fail((mk::set_error("t2m plan: count copy failed"), MK_ERR_HIP));
     int64_t pos = s0;
     for (int64_t t = 0; t < ntiles; t++) {
-      int64_t c = h_cnt[t];
+      int64_t c = (int64_t)h_cnt[ntiles + t] - h_cnt[t];
       for (int64_t o = 0; o < c; o += item_samples) {
         T2MItem it;
         it.seg_base = s0;

@@ -105,17 +105,28 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
         dst[1] = v1;
       }
     }
-    // B tile: 32 rows (k) x 128 columns (n), FP32 -> hi/lo, stored [n][k]
-#pragma unroll 4
-    for (int idx = tid; idx < BK * BN; idx += 256) {
-      int kk = idx >> 7, nn = idx & (BN - 1);
-      int gk = k0 + kk, gn = n0 + nn;
-      float x = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
-      __bf16 h, m, l;
-      split3_bf16(x, &h, &m, &l);
-      Bs[0][nn][kk] = h;
-      Bs[1][nn][kk] = m;
-      if (NPL == 3) Bs[NPL - 1][nn][kk] = l;
+    // B tile: 32 rows (k) x 128 columns (n), FP32 -> bf16 pieces, stored [n][k].
+    // All 16 loads of a thread are issued before any is used.
+    {
+      constexpr int PER = BK * BN / 256;
+      float xv[PER];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int idx = threadIdx.x + u * 256;
+        int kk = idx >> 7, nn = idx & (BN - 1);
+        int gk = k0 + kk, gn = n0 + nn;
+        xv[u] = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        int idx = threadIdx.x + u * 256;
+        int kk = idx >> 7, nn = idx & (BN - 1);
+        __bf16 h, m, l;
+        split3_bf16(xv[u], &h, &m, &l);
+        Bs[0][nn][kk] = h;
+        Bs[1][nn][kk] = m;
+        if (NPL == 3) Bs[NPL - 1][nn][kk] = l;
+      }
     }
     __syncthreads();
 #pragma unroll

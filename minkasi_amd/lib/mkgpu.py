@@ -69,6 +69,7 @@ _SIGS = {
     "mk_filt_plan_borrow_spectrum": ([P, I, P, I], I),
     "mk_filt_plan_exec_f32": ([P, P, P, P], I),
     "mk_filt_plan_exec_f64": ([P, P, P, P], I),
+    "mk_filt_plan_release_work": ([P, P], I),
     "mk_filt_plan_info": ([P, P], I),
     "mk_filt_plan_geometry": ([P, P], I),
     "mk_filt_plan_destroy": ([P], I),

@@ -85,6 +85,9 @@ class FiltPlan(_Handle):
     def set_halve(self, halve):
         mkgpu.check("mk_filt_plan_set_halve", self.h, int(bool(halve)))
 
+    def release_work(self):
+        mkgpu.check("mk_filt_plan_release_work", self.h, dev.stream_ptr())
+
     def borrow(self, seg, src, src_seg=0):
         mkgpu.check("mk_filt_plan_borrow_spectrum", self.h, int(seg), src.h, int(src_seg))
         self._keep.append(src)

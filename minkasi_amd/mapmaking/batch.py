@@ -72,6 +72,8 @@ class BatchedDot:
         self.rot_v = RotPlan([(tod.noise.v_split, n, o, n, o, n) for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
         self.rot_vT = RotPlan([(tod.noise.vT_split, n, o, n, o, n)
                                for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
+        for tod in self.tods:  # per-TOD filter work buffers are not needed on the batched path
+            tod.noise.filt32.release_work()
         self.graph = None
         self._gx = self._gout = None
 
@@ -112,6 +114,28 @@ class BatchedDot:
             self.stages(x, out, accumulate)
         self.graph, self._gx, self._gout, self._gacc = g, x, out, accumulate
         return g
+
+    def rhs(self, out, accumulate=True):
+        """out (+)= sum over TODs of P^T N^-1 d (TodVec.make_rhs,
+        minkasi/tods/core.py:1028-1047) with the batched kernels."""
+        n = self.nsamp
+        a, b = self.buf_a[:n], self.buf_b[:n]
+        for tod, o, (ndet, nn) in zip(self.tods, self.offs, self.sizes):
+            d = tod.get_data()
+            d = d if isinstance(d, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(d))
+            a[o:o + ndet * nn].copy_(d.reshape(-1).to(a.device, torch.float32))
+        self.rot_v.exec(a, b)
+        self.filt.exec(b, b)
+        self.rot_vT.exec(b, a)
+        self.t2m.exec(out.reshape(-1), a, accumulate)
+        return out
+
+    def hits(self, out, accumulate=True):
+        """Hit counts through the same deterministic plan (make_hits)."""
+        n = self.nsamp
+        self.buf_a[:n].fill_(1.0)
+        self.t2m.exec(out.reshape(-1), self.buf_a[:n], accumulate)
+        return out
 
     def stage_times(self, x, out, reps=3):
         """Per-stage device time (ms) with events on the launch stream."""

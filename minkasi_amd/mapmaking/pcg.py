@@ -142,74 +142,105 @@ def _run_pcg_generic(b, x0, tods, prior, precon, mask, maxiter, outroot, save_it
     return x
 
 
+class DevicePCG:
+    """PCG over one FP32 map with device-resident FP64 scalars, in the
+    update order of pcg.py:143-170.  `precon` is None, a Mapset holding the
+    Jacobi weight map (fused update kernel), or an object with
+    apply_to(r, z) (e.g. the learned preconditioner)."""
+
+    def __init__(self, op, b_map, precon=None):
+        self.op = op
+        self.b = b_map
+        self.n = b_map.numel()
+        self.S = dev.scratch()
+        self.r = torch.empty_like(b_map)
+        self.z = torch.empty_like(b_map)
+        self.p = torch.empty_like(b_map)
+        self.Ap = torch.empty_like(b_map)
+        self.jacobi = precon is not None and _single_f32_map(precon)
+        self.w = precon.maps[0].map if self.jacobi else None
+        self.precon = precon
+        self.x = None
+        self.bb = None
+
+    def _M(self, rv, zv):
+        if self.precon is None:
+            zv.copy_(rv)
+        elif self.jacobi:
+            mkgpu.check("mk_mul_f32", zv.data_ptr(), self.w.data_ptr(), rv.data_ptr(), self.n, dev.stream_ptr())
+        else:
+            self.precon.apply_to(rv, zv)
+
+    def init(self, x_map):
+        """x <- x0 (in place), r = b - A x, z = M r, p = z, zr = r.z."""
+        n, sc, work, sp = self.n, self.S.sc, self.S.work.data_ptr(), dev.stream_ptr()
+        self.x = x_map
+        sc.zero_()
+        self.op.apply(self.x, self.Ap, accumulate=False)
+        self.r.copy_(self.b)
+        mkgpu.check("mk_axpy_f32", self.r.data_ptr(), self.Ap.data_ptr(), -1.0, n, sp)
+        self._M(self.r, self.z)
+        mkgpu.check("mk_pcg_init", self.r.data_ptr(), self.z.data_ptr(), self.p.data_ptr(), n, sc.data_ptr(), work,
+                    sp)
+        mkgpu.check("mk_dot_f32", self.b.data_ptr(), self.b.data_ptr(), n, sc.data_ptr(), work, sp)
+        self.bb = float(sc[mkgpu.SC_DOT].item())
+
+    def step(self):
+        n, sc, work, sp = self.n, self.S.sc, self.S.work.data_ptr(), dev.stream_ptr()
+        x, r, z, p, Ap = self.x, self.r, self.z, self.p, self.Ap
+        self.op.apply(p, Ap, accumulate=False)
+        mkgpu.check("mk_pcg_alpha", p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(), work, sp)
+        if self.jacobi:
+            mkgpu.check("mk_pcg_xrz_jacobi", x.data_ptr(), r.data_ptr(), z.data_ptr(), p.data_ptr(), Ap.data_ptr(),
+                        self.w.data_ptr(), n, sc.data_ptr(), work, sp)
+        else:
+            mkgpu.check("mk_pcg_xr", x.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(),
+                        work, sp)
+            self._M(r, z)
+            mkgpu.check("mk_pcg_beta", r.data_ptr(), z.data_ptr(), n, sc.data_ptr(), work, sp)
+        mkgpu.check("mk_pcg_p", p.data_ptr(), z.data_ptr(), n, sc.data_ptr(), sp)
+
+    def scalars(self):
+        """Host copy of the scalar block (synchronises)."""
+        s = self.S.sc.cpu().numpy()
+        if s[mkgpu.SC_FLAG] != 0:
+            raise PCGBreakdown("PCG breakdown (flag %g, pAp %r)" % (s[mkgpu.SC_FLAG], s[mkgpu.SC_PAP]))
+        return s
+
+    def rel_res(self, s=None):
+        s = self.scalars() if s is None else s
+        return float(np.sqrt(max(s[mkgpu.SC_RR], 0.0) / self.bb)) if self.bb > 0 else float("nan")
+
+    def true_rel_res(self):
+        """||b - A x|| / ||b|| with the same operator, FP64 norm."""
+        rt = torch.empty_like(self.b)
+        self.op.apply(self.x, rt, accumulate=False)
+        d = self.b.to(torch.float64) - rt.to(torch.float64)
+        return float(torch.linalg.vector_norm(d).item() / np.sqrt(self.bb))
+
+
 def _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
                     true_residual_every, history, verbose):
     """Device-scalar PCG with the same update order as pcg.py:143-170."""
     op = tods.batched(b)
-    S = dev.scratch()
-    sp = dev.stream_ptr
-    bmap = b.maps[0].map
-    n = bmap.numel()
     x = x0.copy()
-    xm = x.maps[0].map
-    r = torch.empty_like(bmap)
-    z = torch.empty_like(bmap)
-    p = torch.empty_like(bmap)
-    Ap = torch.empty_like(bmap)
-    sc = S.sc
-    sc.zero_()
-    work = S.work.data_ptr()
-    jacobi = precon is not None and _single_f32_map(precon)
-    w = precon.maps[0].map if jacobi else None
-
-    def apply_precon(rv, zv):
-        if precon is None:
-            zv.copy_(rv)
-        elif jacobi:
-            mkgpu.check("mk_mul_f32", zv.data_ptr(), w.data_ptr(), rv.data_ptr(), n, sp())
-        else:
-            precon.apply_to(rv, zv)
-
-    t1 = time.time()
-    op.apply(xm, Ap, accumulate=False)
-    r.copy_(bmap)
-    mkgpu.check("mk_axpy_f32", r.data_ptr(), Ap.data_ptr(), -1.0, n, sp())
-    apply_precon(r, z)
-    mkgpu.check("mk_pcg_init", r.data_ptr(), z.data_ptr(), p.data_ptr(), n, sc.data_ptr(), work, sp())
-    mkgpu.check("mk_dot_f32", bmap.data_ptr(), bmap.data_ptr(), n, sc.data_ptr(), work, sp())
-    bb = float(sc[mkgpu.SC_DOT].item())
+    solver = DevicePCG(op, b.maps[0].map, precon)
+    solver.init(x.maps[0].map)
     nsamp = tods.get_nsamp()
     tloop = time.time()
     it_done = 0
     for i in range(maxiter):
         t1 = time.time()
-        op.apply(p, Ap, accumulate=False)
-        mkgpu.check("mk_pcg_alpha", p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(), work, sp())
-        if jacobi:
-            mkgpu.check("mk_pcg_xrz_jacobi", xm.data_ptr(), r.data_ptr(), z.data_ptr(), p.data_ptr(), Ap.data_ptr(),
-                        w.data_ptr(), n, sc.data_ptr(), work, sp())
-        else:
-            mkgpu.check("mk_pcg_xr", xm.data_ptr(), r.data_ptr(), p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(),
-                        work, sp())
-            apply_precon(r, z)
-            mkgpu.check("mk_pcg_beta", r.data_ptr(), z.data_ptr(), n, sc.data_ptr(), work, sp())
-        mkgpu.check("mk_pcg_p", p.data_ptr(), z.data_ptr(), n, sc.data_ptr(), sp())
+        solver.step()
         it_done = i + 1
-        need_host = verbose or tol is not None or history is not None
-        if need_host:
-            s = sc.cpu().numpy()
+        if verbose or tol is not None or history is not None:
+            s = solver.scalars()
             t3 = time.time()
-            if s[mkgpu.SC_FLAG] != 0:
-                raise PCGBreakdown("PCG breakdown at iteration %d (flag %g, pAp %r)" % (i, s[mkgpu.SC_FLAG],
-                                                                                      s[mkgpu.SC_PAP]))
-            rel = float(np.sqrt(s[mkgpu.SC_RR] / bb)) if bb > 0 else float("nan")
+            rel = solver.rel_res(s)
             rec = dict(it=i, zr=float(s[mkgpu.SC_ZR]), alpha=float(s[mkgpu.SC_ALPHA]), beta=float(s[mkgpu.SC_BETA]),
                        rel_res=rel, t_iter=t3 - t1)
             if true_residual_every and (i + 1) % true_residual_every == 0:
-                rt = torch.empty_like(bmap)
-                op.apply(xm, rt, accumulate=False)
-                rt = bmap.to(torch.float64) - rt.to(torch.float64)
-                rec["true_rel_res"] = float(torch.linalg.vector_norm(rt).item() / np.sqrt(bb))
+                rec["true_rel_res"] = solver.true_rel_res()
             if history is not None:
                 history.append(rec)
             if verbose and myrank == 0:

@@ -36,6 +36,15 @@ def make_hits(todvec, map, do_weights=False):
     import numpy as np
     hits = map.copy()
     hits.clear()
+    if not do_weights and not any("mask" in t.info for t in todvec.tods):
+        from ..maps.mapset import Mapset
+        ms = Mapset()
+        ms.maps.append(hits)
+        ms.nmap = 1
+        b = todvec.batched(ms)
+        if b is not None:
+            b.hits(hits.map, accumulate=True)
+            return hits
     for tod in todvec.tods:
         shape = tod.get_data_dims()
         if do_weights:

@@ -287,6 +287,10 @@ class TodVec:
         """core.py:1028-1047."""
         if do_clear:
             mapset.clear()
+        b = self.batched(mapset)
+        if b is not None:
+            b.rhs(mapset.maps[0].map, accumulate=True)
+            return
         for tod in self.tods:
             dat_filt = tod.apply_noise()
             for map in mapset.maps:

@@ -1,0 +1,73 @@
+"""Builds the synthetic mapmaking problems of the benchmark configurations on
+the GPU: TODs, map, pixelization, noise models, hits, Jacobi preconditioner,
+right-hand side and the batched operator, following the canonical sequence
+of minkasi's examples/minkasi_mpi_example.py:35-112."""
+import time
+
+import numpy as np
+import torch
+
+from . import device as dev
+from .synth import ARCSEC, CONFIGS, ScanSpec, make_tod_info, tod_shapes
+
+
+class Problem:
+    pass
+
+
+def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False):
+    from .maps.mapset import Mapset
+    from .maps.skymap import SkyMap
+    from .mapmaking.noise import NoiseSmoothedSVD
+    from .mapmaking.tod2map import make_hits
+    from .tods.core import Tod, TodVec
+
+    c = CONFIGS[config_id]
+    shapes = shapes or tod_shapes(config_id, ntod)
+    spec = ScanSpec(**c["spec"])
+    dtype = np.float32 if c["dtype"] == "float32" else np.float64
+    P = Problem()
+    P.config_id, P.shapes = config_id, shapes
+    t0 = time.time()
+    todvec = TodVec()
+    for i, (ndet, n) in enumerate(shapes):
+        info = make_tod_info(config_id, seed_base + i, ndet, n, spec, dtype=dtype, backend="torch",
+                             want_radec=False)
+        todvec.add_tod(Tod(info))
+    torch.cuda.synchronize()
+    P.t_generate = time.time() - t0
+    t0 = time.time()
+    lims = todvec.lims()
+    N = c["npix"]
+    m = SkyMap(lims, c["pix"] * ARCSEC, nx=N, ny=N,
+               dtype=torch.float32 if dtype == np.float32 else torch.float64)
+    for tod in todvec.tods:
+        m.get_pix(tod)
+    torch.cuda.synchronize()
+    P.t_pixelize = time.time() - t0
+    t0 = time.time()
+    for k, tod in enumerate(todvec.tods):
+        tod.set_noise(NoiseSmoothedSVD, fwhm=c["fwhm"])
+        if verbose and k % 16 == 0:
+            print("noise model", k, "of", len(todvec.tods), flush=True)
+    torch.cuda.synchronize()
+    P.t_noise = time.time() - t0
+    t0 = time.time()
+    mapset = Mapset()
+    mapset.add_map(m)
+    op = todvec.batched(mapset)
+    torch.cuda.synchronize()
+    P.t_plans = time.time() - t0
+    t0 = time.time()
+    hits = make_hits(todvec, m)
+    rhs = mapset.copy()
+    rhs.clear()
+    todvec.make_rhs(rhs)
+    pre = mapset.copy()
+    h = hits.map
+    pre.maps[0].map.copy_(torch.where(h > 0, 1.0 / torch.where(h > 0, h, torch.ones_like(h)), torch.zeros_like(h)))
+    torch.cuda.synchronize()
+    P.t_rhs = time.time() - t0
+    P.todvec, P.map, P.mapset, P.hits, P.rhs, P.precon, P.op = todvec, m, mapset, hits, rhs, pre, op
+    P.nsamp = sum(a * b for a, b in shapes)
+    return P
