@@ -259,6 +259,61 @@ int mk_smooth_rows_f64(const double *x, double *out, float *out32,
                        int64_t nrow, int64_t n, const double *g, int32_t t_lo,
                        int32_t t_hi, int32_t flags, void *stream);
 
+/* x -= (u.x * inv_uu) u with u.x accumulated in FP64 into sc[MK_SC_DOT]:
+ * removes the component of x along u (inv_uu = 1/u.u).  With u the
+ * indicator of hit pixels it projects A.p off the null mode of A (the
+ * constant on hit pixels); an exact no-op in exact arithmetic, it keeps FP32
+ * PCG consistent.  No reference counterpart (minkasi/mapmaking/pcg.py:143-170
+ * runs in FP64). */
+int mk_project_out_f32(float *x, const float *u, int64_t n, double inv_uu,
+                       double *sc, void *work, void *stream);
+
+/* ---- Learned preconditioner (no reference counterpart; it plugs into the
+ * reference's preconditioner hook `precon * r`, minkasi/mapmaking/pcg.py:
+ * 110-111,157-158).  M = S (alpha I + K^T K) S, S = diag(sqrt(1/hits)),
+ * K = chain of linear dilated 3x3 convolutions.  Planar FP32 activations
+ * [B][C][H][W]; C is 1 or 16; weights w[Cout][Cin][9], tap d = 3*di + dj
+ * at offset (dil*(di-1), dil*(dj-1)); zero padding. */
+/* y (+)= conv(x, w) with dilation dil. */
+int mk_conv3x3_f32(const float *x, int32_t B, int32_t Cin, int32_t H,
+                   int32_t W, const float *w, int32_t Cout, int32_t dil,
+                   float *y, int32_t accumulate, void *stream);
+/* wt[c][o][8-d] = w[o][c][d]: weights of the adjoint layer. */
+int mk_conv3x3_adjoint_weights(const float *w, int32_t Co, int32_t Ci,
+                               float *wt, void *stream);
+/* g[o][c][d] (+)= sum_{b,p} a[b,o,p] x[b,c,p+off_d]  (weight gradient). */
+int64_t mk_conv3x3_wgrad_workspace_bytes(int32_t B, int32_t Co, int32_t Ci,
+                                         int32_t H, int32_t W, int32_t dil);
+int mk_conv3x3_wgrad_f32(const float *a, int32_t Co, const float *x,
+                         int32_t Ci, int32_t B, int32_t H, int32_t W,
+                         int32_t dil, float *g, int32_t accumulate,
+                         void *work, int64_t work_bytes, void *stream);
+/* z = s * (exp(log_alpha[0]) * u + g); g may be NULL (Jacobi-like). */
+int mk_precon_finish_f32(float *z, const float *s, const float *log_alpha,
+                         const float *u, const float *g, int64_t n,
+                         void *stream);
+/* Training loss L = sum m (s (alpha u + g) - e)^2 with q = dL/d(alpha u + g)
+ * written per pixel and sc[0] = L, sc[1] = dL/dalpha (FP64, device). */
+int64_t mk_precon_loss_workspace_bytes(void);
+int mk_precon_loss_f32(const float *g, const float *u, const float *s,
+                       const float *e, const float *m, const float *log_alpha,
+                       int64_t n, float *q, double *sc, void *work,
+                       int64_t work_bytes, void *stream);
+/* Energy-norm training loss: z = M r (full map), az = A z from the mapmaking
+ * operator, e the error with A e = r.  sc[0] = (z - e).(az - r)
+ * = ||z - e||_A^2, q = 2 s (az - r) (the gradient with respect to
+ * t = alpha u + K^T K u, z = s t), sc[1] = sum q u = dL/dalpha. */
+int mk_precon_aloss_f32(const float *z, const float *az, const float *r,
+                        const float *e, const float *s, const float *u,
+                        int64_t n, float *q, double *sc, void *work,
+                        int64_t work_bytes, void *stream);
+/* Adam step on a flat parameter vector whose last element is log(alpha);
+ * that element's gradient is alpha * dalpha[0].  gscale multiplies every
+ * gradient. */
+int mk_adam_f32(float *p, const float *g, float *m, float *v, int32_t n,
+                const double *dalpha, float lr, float beta1, float beta2,
+                float eps, int32_t step, float gscale, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -243,3 +243,25 @@ extern "C" int mk_pcg_init(const float* r, const float* z, float* p, int64_t n, 
   A.r = const_cast<float*>(r); A.z = const_cast<float*>(z); A.p = p; A.n = n;
   return run<kInit>(A, sc, work, stream);
 }
+
+namespace {
+__global__ void __launch_bounds__(kBlock) project_kernel(float* __restrict__ x, const float* __restrict__ u,
+                                                         const double* __restrict__ sc, double inv_uu, int64_t n) {
+  const double c = sc[MK_SC_DOT] * inv_uu;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = (float)((double)x[i] - c * (double)u[i]);
+}
+}  // namespace
+
+extern "C" int mk_project_out_f32(float* x, const float* u, int64_t n, double inv_uu, double* sc, void* work,
+                                  void* stream) {
+  MK_REQUIRE(n >= 0 && (n == 0 || (x && u)) && sc && work, MK_ERR_ARG, "bad arguments");
+  if (n == 0) return MK_OK;
+  Args A{};
+  A.a = x; A.b = u; A.n = n;
+  int st = run<kDot>(A, sc, work, stream);
+  if (st) return st;
+  project_kernel<<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(x, u, sc, inv_uu, n);
+  return mk::check_launch("project_kernel");
+}

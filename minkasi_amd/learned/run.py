@@ -1,0 +1,341 @@
+"""Harvest, train and evaluate the learned preconditioner on one GPU.
+
+    python -m minkasi_amd.learned.run --seeds 32 --ntod 64 --pairs 5000 \
+        --out minkasi_amd/learned/weights_c4.npz [--eval]
+
+Training problems use the 64-TOD, 2048^2 distribution with TOD seeds and
+shapes disjoint from the benchmark problem (seed_base = 10000 + 1000 * seed,
+shape seed 20000 + seed).  --eval then solves the benchmark problem
+(seed_base 0) to ||r||/||b|| < tol with Jacobi and with the trained network
+and prints one JSON line.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+import torch
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def parse(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--seeds", type=int, default=32)
+    ap.add_argument("--ntod", type=int, default=64, help="TODs per training problem")
+    ap.add_argument("--pairs", type=int, default=5000)
+    ap.add_argument("--crop", type=int, default=512)
+    ap.add_argument("--dils", default="1,3,9,27")
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--init-frac", type=float, default=0.3, help="initial K^T K gain relative to alpha")
+    ap.add_argument("--max-iter", type=int, default=150)
+    ap.add_argument("--tol-final", type=float, default=5e-8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--eval", action="store_true")
+    ap.add_argument("--eval-ntod", type=int, default=None)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--maxiter", type=int, default=400)
+    return ap.parse_args(argv)
+
+
+class Dataset:
+    """Crops (r, e, s, m) on the device, split into training and validation."""
+
+    def __init__(self, R, E, S, M, val_frac=0.1, seed=0):
+        N = R.shape[0]
+        perm = torch.from_numpy(np.random.default_rng(seed).permutation(N)).to(R.device)
+        nval = max(1, int(N * val_frac)) if N > 10 else 0
+        self.val = [t[perm[:nval]] for t in (R, E, S, M)]
+        self.tr = [t[perm[nval:]] for t in (R, E, S, M)]
+        self.N = self.tr[0].shape[0]
+        self.crop = R.shape[1]
+
+    def set_margin(self, margin):
+        """Loss weights restricted to pixels at least `margin` inside each crop."""
+        C = self.crop
+        inner = torch.zeros((C, C), dtype=torch.float32, device=self.tr[3].device)
+        if C > 2 * margin:
+            inner[margin:C - margin, margin:C - margin] = 1.0
+        if not hasattr(self, "_m_full"):
+            self._m_full = (self.tr[3], self.val[3])
+        self.tr[3] = self._m_full[0] * inner
+        self.val[3] = self._m_full[1] * inner
+
+    def alpha_fit(self):
+        """Closed-form alpha for K = 0: argmin sum m (alpha s^2 r - e)^2, and its loss per crop."""
+        R, E, S, M = (t.double() for t in self.tr)
+        s2r = S * S * R
+        a = float((M * s2r * E).sum() / (M * s2r * s2r).sum())
+        loss = float((M * (a * s2r - E) ** 2).sum() / self.N)
+        return a, loss
+
+
+def build_dataset(config, seeds, ntod, pairs, crop, margin, rng, max_iter, tol_final, shapes_seed0=20000,
+                  seed_base0=10000, weight="hits"):
+    from ..synth import tod_shapes
+    from ..workloads import build_problem
+    from .harvest import harvest
+    per_seed = int(np.ceil(pairs / seeds))
+    parts = []
+    t0 = time.time()
+    for s in range(seeds):
+        shapes = tod_shapes(config, ntod, seed=shapes_seed0 + s)
+        P = build_problem(config, shapes=shapes, seed_base=seed_base0 + 1000 * s)
+        (R, E, S, M, ks), rel = harvest(P, per_seed, crop, margin, rng, max_iter=max_iter, tol_final=tol_final,
+                                       weight=weight)
+        parts.append((R, E, S, M))
+        log("harvest seed %d: best rel res %.2e after %d iterations, %d crops from iterations %d..%d (%.0f s)" % (
+            s, rel[-1], len(rel) - 1, R.shape[0], ks.min(), ks.max(), time.time() - t0))
+        del P
+        torch.cuda.empty_cache()
+    out = [torch.cat([p[i] for p in parts]) for i in range(4)]
+    return Dataset(*out)
+
+
+def train(ds, dils, steps, lr, batch, seed=0, init_frac=0.3, log_every=100):
+    """Adam on the training crops; returns (params numpy, history, meta).
+    The loss ignores pixels within the reach of K^T K (2 * sum(dils)) of a
+    crop's border, where the crop lacks inputs the full map has."""
+    from .precon import init_params, param_layout
+    from .train import Trainer
+    ds.set_margin(2 * sum(dils))
+    alpha0, base_loss = ds.alpha_fit()
+    lay, _ = param_layout(len(dils))
+    p0 = init_params(dils, seed=seed, scale=1.0, log_alpha=np.log(alpha0))
+    gain = (init_frac * alpha0) ** (1.0 / (2 * len(dils)))
+    for off, co, ci in lay:
+        p0[off:off + co * ci * 9] *= gain / np.sqrt(9.0 * ci)
+    B, C = batch, ds.crop
+    tr = Trainer(p0, dils, B, C, C, lr=lr)
+    R, E, S, M = ds.tr
+    bufs = [torch.empty((B, 1, C, C), dtype=torch.float32, device=R.device) for _ in range(4)]
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    t1 = time.time()
+    hist = []
+    for it in range(steps):
+        idx = torch.randint(0, ds.N, (B,), generator=g).to(R.device)
+        for src, dst in zip((R, E, S, M), bufs):
+            torch.index_select(src, 0, idx, out=dst.view(B, C, C))
+        if it == int(steps * 0.6) or it == int(steps * 0.85):
+            tr.lr *= 0.3
+        tr.step(*bufs, gscale=1.0 / B)
+        if log_every and (it % log_every == 0 or it == steps - 1):
+            l = tr.loss() / B
+            hist.append((it, l))
+            log("  step %d loss/crop %.4f (alpha-only fit %.4f) alpha %.4g  %.1f ms/step" % (
+                it, l, base_loss, float(np.exp(tr.params[-1].item())), 1000 * (time.time() - t1) / (it + 1)))
+    torch.cuda.synchronize()
+    params = tr.params.cpu().numpy()
+    vloss = val_loss(ds, params, dils, B)
+    meta = dict(dils=list(dils), crops=ds.N, crop=C, steps=steps, batch=B, lr=lr, init_frac=init_frac,
+                alpha_fit=alpha0, alpha_fit_loss=base_loss, val_loss=vloss, train_s=time.time() - t1)
+    log("  trained: validation loss/crop %.4f (alpha-only fit on training crops %.4f)" % (vloss, base_loss))
+    return params, hist, meta
+
+
+def energy_pairs(P, max_iter, tol_final, min_ratio=10.0):
+    """Full-map (r_k, e_k) pairs of one problem, each scaled by
+    1/sqrt(e_k . r_k) = 1/||e_k||_A so that M = 0 has unit loss."""
+    from .harvest import run_jacobi_history
+    xs, rs, rel = run_jacobi_history(P, max_iter, tol_final)
+    hit = P.hits.map > 0
+    nhit = float(hit.sum().item())
+    xstar = xs[-1]
+    pairs = []
+    for k in range(len(xs) - 1):
+        if rel[k] < min_ratio * rel[-1]:
+            continue
+        e = xstar - xs[k]
+        mean = float((e.to(torch.float64) * hit).sum().item()) / nhit
+        e = torch.where(hit, e - mean, torch.zeros_like(e))
+        er = float((e.to(torch.float64) * rs[k].to(torch.float64)).sum().item())
+        if not er > 0:
+            continue
+        c = 1.0 / np.sqrt(er)
+        pairs.append((rs[k] * c, e * c, k))
+    del xs, rs
+    return pairs, rel
+
+
+def train_energy(config, seeds, ntod, steps_per_seed, dils, lr, seed=0, init_frac=0.01, max_iter=150,
+                 tol_final=5e-8, shapes_seed0=20000, seed_base0=10000, log_every=25, warmup=50, init=None):
+    """Energy-norm training on harvested full-map pairs, one problem at a
+    time (its operator supplies A z).  Returns (params, history, meta)."""
+    from ..synth import tod_shapes
+    from ..workloads import build_problem
+    from .precon import init_params, param_layout
+    from .train import Trainer
+    rng = np.random.default_rng(seed)
+    lay, _ = param_layout(len(dils))
+    tr = None
+    hist = []
+    total = seeds * steps_per_seed
+    t0 = time.time()
+    npairs = 0
+    for si in range(seeds):
+        shapes = tod_shapes(config, ntod, seed=shapes_seed0 + si)
+        P = build_problem(config, shapes=shapes, seed_base=seed_base0 + 1000 * si)
+        pairs, rel = energy_pairs(P, max_iter, tol_final)
+        npairs += len(pairs)
+        hits = P.hits.map
+        hf = hits.to(torch.float64)
+        smap = torch.where(hits > 0, torch.rsqrt(torch.where(hits > 0, hf, torch.ones_like(hf))),
+                           torch.zeros_like(hf)).to(torch.float32).contiguous()
+        op = P.op
+        if tr is None:
+            # alpha of the best pure Jacobi step on the first pair: (z.r)/(z.Az), z = J r
+            r0 = pairs[0][0]
+            z0 = smap * smap * r0
+            az0 = torch.empty_like(z0)
+            op.apply(z0, az0, accumulate=False)
+            alpha0 = float((z0.double() * r0.double()).sum() / (z0.double() * az0.double()).sum())
+            if init is not None:
+                p0 = np.array(init, dtype=np.float32, copy=True)
+            else:
+                p0 = init_params(dils, seed=seed, scale=1.0, log_alpha=np.log(alpha0))
+                gain = (init_frac * alpha0) ** (1.0 / (2 * len(dils)))
+                for off, co, ci in lay:
+                    p0[off:off + co * ci * 9] *= gain / np.sqrt(9.0 * ci)
+            tr = Trainer(p0, dils, 1, hits.shape[0], hits.shape[1], lr=lr)
+            # energy loss of the pure Jacobi step on the first pairs (the reference level)
+            jl = []
+            for r_, e_, _ in pairs[:: max(1, len(pairs) // 8)]:
+                zj = alpha0 * smap * smap * r_
+                azj = torch.empty_like(zj)
+                op.apply(zj, azj, accumulate=False)
+                jl.append(float(((zj - e_).double() * (azj - r_).double()).sum()))
+            jac_loss = float(np.mean(jl))
+            tr.z = torch.empty(hits.shape, dtype=torch.float32, device=hits.device)
+            tr.az = torch.empty_like(tr.z)
+            log("  alpha of the Jacobi step: %.4g; its energy loss %.4f" % (alpha0, jac_loss))
+        op.capture(tr.z, tr.az, accumulate=False)
+
+        def apply_A(z, out):
+            op.apply(z, out, accumulate=False)
+
+        acc, nacc = 0.0, 0
+        for st in range(steps_per_seed):
+            it = si * steps_per_seed + st
+            tr.lr = lr * min(1.0, (it + 1) / float(warmup)) * (0.3 if it >= int(total * 0.6) else 1.0) * (
+                0.3 if it >= int(total * 0.85) else 1.0)
+            r, e, k = pairs[int(rng.integers(len(pairs)))]
+            tr.step_energy(r, e, smap, apply_A)
+            if log_every and (st % log_every == log_every - 1 or st == steps_per_seed - 1):
+                l = tr.loss()
+                acc += l
+                nacc += 1
+                hist.append((it, l))
+        log("  seed %d: %d pairs (best rel res %.1e after %d its); mean energy loss %.4f; alpha %.4g; %.0f s" % (
+            si, len(pairs), rel[-1], len(rel) - 1, acc / max(nacc, 1), float(np.exp(tr.params[-1].item())),
+            time.time() - t0))
+        op.graph = None
+        del P, pairs, op
+        torch.cuda.empty_cache()
+    params = tr.params.cpu().numpy()
+    meta = dict(loss="energy", dils=list(dils), seeds=seeds, ntod=ntod, pairs=npairs, steps=total, lr=lr,
+                init_frac=init_frac, train_s=time.time() - t0, jacobi_step_loss=jac_loss,
+                final_loss=float(np.mean([h[1] for h in hist[-8:]])) if hist else None)
+    return params, hist, meta
+
+
+def val_loss(ds, params, dils, B):
+    """Loss per crop on the validation crops (forward only)."""
+    from .train import Trainer
+    R, E, S, M = ds.val
+    n = R.shape[0]
+    if n == 0:
+        return float("nan")
+    C = ds.crop
+    tr = Trainer(params, dils, 1, C, C)
+    tot = 0.0
+    for i in range(n):
+        tr.forward_backward(*(t[i:i + 1].reshape(1, 1, C, C).contiguous() for t in (R, E, S, M)))
+        tot += tr.loss()
+    return tot / n
+
+
+def scale_ktk(params, dils, c):
+    """Parameters with K^T K scaled by c (every layer by c^(1/(2L))), alpha kept."""
+    from .precon import param_layout
+    p = np.array(params, dtype=np.float32, copy=True)
+    lay, _ = param_layout(len(dils))
+    f = np.float32(c ** (1.0 / (2 * len(dils))))
+    for off, co, ci in lay:
+        p[off:off + co * ci * 9] *= f
+    return p
+
+
+def calibrate(P, params, dils, tol, maxiter, factors=(0.25, 0.5, 1.0, 2.0, 4.0)):
+    """Picks the K^T K scale (relative to alpha) that needs the fewest PCG
+    iterations to tol on problem P (a held-out training problem): CG is
+    invariant to the scale of M, not to the balance of its two terms."""
+    from ..workloads import solve_to_tol
+    from .precon import LearnedPrecon
+    best = None
+    rows = []
+    for c in factors:
+        pc = scale_ktk(params, dils, c)
+        res = solve_to_tol(P, LearnedPrecon(P.hits, pc, dils), tol, maxiter)
+        its = res["iters"] if res["reached"] else maxiter + 1
+        rows.append((c, its))
+        log("  calibrate: K^T K scale %.3g -> %d iterations" % (c, its))
+        if best is None or its < best[1]:
+            best = (c, its)
+    return scale_ktk(params, dils, best[0]), best[0], rows
+
+
+def evaluate(P, params, dils, tol, maxiter, jac=None):
+    from ..workloads import map_distance, solve_to_tol
+    from .precon import LearnedPrecon
+    if jac is None:
+        jac = solve_to_tol(P, None, tol, maxiter)
+    lp = LearnedPrecon(P.hits, params, dils)
+    lrn = solve_to_tol(P, lp, tol, maxiter)
+    dist = map_distance(P, lrn["x"], jac["x"])
+    out = dict(learned=dict((k, v) for k, v in lrn.items() if k not in ("x", "history")),
+               jacobi=dict((k, v) for k, v in jac.items() if k not in ("x", "history")),
+               map_distance=dist, speedup=jac["seconds"] / lrn["seconds"],
+               iter_ratio=jac["iters"] / max(lrn["iters"], 1))
+    del lrn, lp
+    return out, jac
+
+
+def main(argv=None):
+    args = parse(argv)
+    from ..workloads import build_problem
+    from .precon import save
+    torch.cuda.set_device(0)
+    dils = tuple(int(d) for d in args.dils.split(","))
+    margin = 2 * sum(dils)
+    rng = np.random.default_rng(args.seed)
+    ds = build_dataset(args.config, args.seeds, args.ntod, args.pairs, args.crop, 0, rng, args.max_iter,
+                       args.tol_final)
+    log("dataset: %d training + %d validation crops of %d^2, loss margin %d" % (
+        ds.N, ds.val[0].shape[0], args.crop, margin))
+    params, hist, meta = train(ds, dils, args.steps, args.lr, args.batch, args.seed, args.init_frac)
+    meta.update(seeds=args.seeds, ntod=args.ntod, history=hist)
+    if args.out:
+        save(args.out, params, dils, meta)
+        log("saved", args.out)
+    del ds
+    torch.cuda.empty_cache()
+    res = dict(train=meta)
+    if args.eval:
+        P = build_problem(args.config, ntod=args.eval_ntod)
+        ev, _ = evaluate(P, params, dils, args.tol, args.maxiter)
+        res.update(ev)
+        log("Jacobi: %d its %.2f s; learned: %d its %.2f s; speedup %.2fx; map distance %.2e" % (
+            ev["jacobi"]["iters"], ev["jacobi"]["seconds"], ev["learned"]["iters"], ev["learned"]["seconds"],
+            ev["speedup"], ev["map_distance"]))
+    print(json.dumps(res, default=float))
+
+
+if __name__ == "__main__":
+    main()

@@ -35,6 +35,16 @@ class MkError(RuntimeError):
 
 
 _SIGS = {
+    "mk_conv3x3_f32": ([P, I32, I32, I32, I32, P, I32, I32, P, I32, P], I),
+    "mk_conv3x3_adjoint_weights": ([P, I32, I32, P, P], I),
+    "mk_conv3x3_wgrad_workspace_bytes": ([I32, I32, I32, I32, I32, I32], I64),
+    "mk_conv3x3_wgrad_f32": ([P, I32, P, I32, I32, I32, I32, I32, P, I32, P, I64, P], I),
+    "mk_precon_finish_f32": ([P, P, P, P, P, I64, P], I),
+    "mk_precon_loss_workspace_bytes": ([], I64),
+    "mk_precon_aloss_f32": ([P, P, P, P, P, P, I64, P, P, P, I64, P], I),
+    "mk_precon_loss_f32": ([P, P, P, P, P, P, I64, P, P, P, I64, P], I),
+    "mk_adam_f32": ([P, P, P, P, I32, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I32,
+                     ctypes.c_float, P], I),
     "mk_last_error": ([], ctypes.c_char_p),
     "mk_version": ([], I),
     "mk_reduce_workspace_bytes": ([], I64),
@@ -59,6 +69,7 @@ _SIGS = {
     "mk_pcg_xrz_jacobi": ([P, P, P, P, P, P, I64, P, P, P], I),
     "mk_pcg_beta": ([P, P, I64, P, P, P], I),
     "mk_pcg_p": ([P, P, I64, P, P], I),
+    "mk_project_out_f32": ([P, P, I64, D, P, P, P], I),
     "mk_pcg_init": ([P, P, P, I64, P, P, P], I),
     "mk_dct1_workspace_bytes": ([I64, I64, I], I64),
     "mk_dct1_f64": ([P, P, I64, I64, P, I64, P], I),

@@ -148,7 +148,7 @@ class DevicePCG:
     Jacobi weight map (fused update kernel), or an object with
     apply_to(r, z) (e.g. the learned preconditioner)."""
 
-    def __init__(self, op, b_map, precon=None):
+    def __init__(self, op, b_map, precon=None, null_mask="auto"):
         self.op = op
         self.b = b_map
         self.n = b_map.numel()
@@ -162,6 +162,27 @@ class DevicePCG:
         self.precon = precon
         self.x = None
         self.bb = None
+        # Null mode of A: the constant on hit pixels.  A.p (and r_0) are
+        # projected off it; an exact no-op in exact arithmetic, it stops FP32
+        # rounding from making the singular system inconsistent, which
+        # otherwise makes the iteration diverge once ||r||/||b|| reaches ~1e-6.
+        if isinstance(null_mask, str) and null_mask == "auto":
+            if self.jacobi:
+                null_mask = self.w > 0
+            elif hasattr(precon, "s") and isinstance(precon.s, torch.Tensor):
+                null_mask = precon.s > 0
+            else:
+                null_mask = None
+        self.u = None
+        if null_mask is not None:
+            self.u = null_mask.to(torch.float32).reshape(b_map.shape).contiguous()
+            cnt = float(self.u.sum().item())
+            self.inv_uu = 1.0 / cnt if cnt > 0 else 0.0
+
+    def _project(self, v):
+        if self.u is not None:
+            mkgpu.check("mk_project_out_f32", v.data_ptr(), self.u.data_ptr(), self.n, self.inv_uu,
+                        self.S.sc.data_ptr(), self.S.work.data_ptr(), dev.stream_ptr())
 
     def _M(self, rv, zv):
         if self.precon is None:
@@ -179,6 +200,7 @@ class DevicePCG:
         self.op.apply(self.x, self.Ap, accumulate=False)
         self.r.copy_(self.b)
         mkgpu.check("mk_axpy_f32", self.r.data_ptr(), self.Ap.data_ptr(), -1.0, n, sp)
+        self._project(self.r)
         self._M(self.r, self.z)
         mkgpu.check("mk_pcg_init", self.r.data_ptr(), self.z.data_ptr(), self.p.data_ptr(), n, sc.data_ptr(), work,
                     sp)
@@ -189,6 +211,7 @@ class DevicePCG:
         n, sc, work, sp = self.n, self.S.sc, self.S.work.data_ptr(), dev.stream_ptr()
         x, r, z, p, Ap = self.x, self.r, self.z, self.p, self.Ap
         self.op.apply(p, Ap, accumulate=False)
+        self._project(Ap)
         mkgpu.check("mk_pcg_alpha", p.data_ptr(), Ap.data_ptr(), n, sc.data_ptr(), work, sp)
         if self.jacobi:
             mkgpu.check("mk_pcg_xrz_jacobi", x.data_ptr(), r.data_ptr(), z.data_ptr(), p.data_ptr(), Ap.data_ptr(),

@@ -137,11 +137,11 @@ CONFIGS = {
 }
 
 
-def tod_shapes(config_id, ntod=None):
-    """(ndet, n) per TOD, drawn from default_rng([config_id, 10**6])."""
+def tod_shapes(config_id, ntod=None, seed=10 ** 6):
+    """(ndet, n) per TOD, drawn from default_rng([config_id, seed])."""
     c = CONFIGS[config_id]
     ntod = c["ntod"] if ntod is None else ntod
-    rng = np.random.default_rng([int(config_id), 10 ** 6])
+    rng = np.random.default_rng([int(config_id), int(seed)])
     out = []
     for _ in range(ntod):
         ndet = int(rng.integers(c["ndet"][0], c["ndet"][1] + 1))

@@ -71,3 +71,47 @@ def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False)
     P.todvec, P.map, P.mapset, P.hits, P.rhs, P.precon, P.op = todvec, m, mapset, hits, rhs, pre, op
     P.nsamp = sum(a * b for a, b in shapes)
     return P
+
+
+def solve_to_tol(P, precon=None, tol=1e-6, maxiter=600, graph=True):
+    """Device PCG from x0 = 0 until the recursive ||r||/||b|| < tol.
+
+    precon: None for the problem's hits-count Jacobi, else an object with
+    apply_to(r, z) (e.g. the learned preconditioner).  Returns a dict with
+    iterations, wall seconds (including init), final recursive and true
+    relative residuals, and the solution tensor."""
+    import time
+    from .mapmaking.pcg import DevicePCG
+    op, b = P.op, P.rhs.maps[0].map
+    solver = DevicePCG(op, b, P.precon if precon is None else precon)
+    x = torch.zeros_like(b)
+    if graph:
+        solver.init(x)
+        op.capture(solver.p, solver.Ap, accumulate=False)
+        x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    solver.init(x)
+    it, rel, hist = 0, 1.0, []
+    while it < maxiter:
+        solver.step()
+        it += 1
+        rel = solver.rel_res()
+        hist.append(rel)
+        if rel < tol:
+            break
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    return dict(iters=it, seconds=dt, rel_res=rel, reached=bool(rel < tol), true_rel_res=solver.true_rel_res(),
+                x=x, history=hist)
+
+
+def map_distance(P, x, y):
+    """||x - y|| / ||y|| over hit pixels after removing the null mode (the
+    mean over hit pixels) from both."""
+    hit = P.hits.map > 0
+    xd = x.to(torch.float64)[hit]
+    yd = y.to(torch.float64)[hit]
+    xd = xd - xd.mean()
+    yd = yd - yd.mean()
+    return float(torch.linalg.vector_norm(xd - yd).item() / torch.linalg.vector_norm(yd).item())

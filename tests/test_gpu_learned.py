@@ -1,0 +1,207 @@
+"""GPU parity of the learned-preconditioner kernels (conv3x3, adjoint weights,
+wgrad, loss, Adam) and of the full forward/backward, through the C ABI,
+against the FP64 oracle (oracle/learned.py).  FP32 on the GPU: tolerances are
+stated per test."""
+import numpy as np
+import pytest
+
+from oracle import learned as OL
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    from minkasi_amd import device
+    from minkasi_amd.lib import mkgpu
+    return torch, device, mkgpu
+
+
+def _t(env, a):
+    torch, device, _ = env
+    return device.to_device(np.ascontiguousarray(a, dtype=np.float32))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.mark.parametrize("cin,cout", [(1, 16), (16, 16), (16, 1), (1, 1)])
+@pytest.mark.parametrize("dil", [1, 3, 27])
+def test_conv3x3(env, cin, cout, dil):
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(cin * 100 + cout + dil)
+    B, H, W = 2, 37, 70
+    x = rng.standard_normal((B, cin, H, W)).astype(np.float32)
+    w = rng.standard_normal((cout, cin, 9)).astype(np.float32)
+    ref = OL.conv3x3(x.astype(np.float64), w.astype(np.float64), dil)
+    xt, wt = _t(env, x), _t(env, w)
+    y = torch.full((B, cout, H, W), 7.0, dtype=torch.float32, device=xt.device)
+    print("conv3x3", cin, cout, dil, flush=True)
+    mkgpu.check("mk_conv3x3_f32", xt.data_ptr(), B, cin, H, W, wt.data_ptr(), cout, dil, y.data_ptr(), 0,
+                device.stream_ptr())
+    got = y.cpu().numpy()
+    assert rel(got, ref) < 2e-6
+    mkgpu.check("mk_conv3x3_f32", xt.data_ptr(), B, cin, H, W, wt.data_ptr(), cout, dil, y.data_ptr(), 1,
+                device.stream_ptr())
+    assert rel(y.cpu().numpy(), 2 * ref) < 2e-6
+
+
+def test_conv3x3_adjoint_weights(env):
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal((16, 1, 9)).astype(np.float32)
+    wt = torch.empty(16 * 9, dtype=torch.float32, device=device.device())
+    w_d = _t(env, w)
+    mkgpu.check("mk_conv3x3_adjoint_weights", w_d.data_ptr(), 16, 1, wt.data_ptr(), device.stream_ptr())
+    assert np.array_equal(wt.cpu().numpy().reshape(1, 16, 9), OL.adjoint_weights(w))
+
+
+def test_conv3x3_rejects_bad_channels(env):
+    torch, device, mkgpu = env
+    x = torch.zeros(64, dtype=torch.float32, device=device.device())
+    with pytest.raises(mkgpu.MkError):
+        mkgpu.check("mk_conv3x3_f32", x.data_ptr(), 1, 3, 4, 4, x.data_ptr(), 16, 1, x.data_ptr() + 4, 0,
+                    device.stream_ptr())
+
+
+@pytest.mark.parametrize("co,ci", [(16, 16), (16, 1)])
+@pytest.mark.parametrize("dil", [1, 9])
+def test_wgrad(env, co, ci, dil):
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(co + ci + dil)
+    B, H, W = 3, 45, 52
+    a = rng.standard_normal((B, co, H, W)).astype(np.float32)
+    x = rng.standard_normal((B, ci, H, W)).astype(np.float32)
+    ref = OL.wgrad(a.astype(np.float64), x.astype(np.float64), dil)
+    nb = int(mkgpu.lib.mk_conv3x3_wgrad_workspace_bytes(B, co, ci, H, W, dil))
+    work = torch.empty(nb // 4 + 1, dtype=torch.float32, device=device.device())
+    g = torch.zeros(co * ci * 9, dtype=torch.float32, device=device.device())
+    print("wgrad", co, ci, dil, flush=True)
+    at, xt = _t(env, a), _t(env, x)  # keep both alive until the kernels have run
+    mkgpu.check("mk_conv3x3_wgrad_f32", at.data_ptr(), co, xt.data_ptr(), ci, B, H, W, dil,
+                g.data_ptr(), 0, work.data_ptr(), work.numel() * 4, device.stream_ptr())
+    assert rel(g.cpu().numpy(), ref) < 1e-5
+    mkgpu.check("mk_conv3x3_wgrad_f32", at.data_ptr(), co, xt.data_ptr(), ci, B, H, W, dil,
+                g.data_ptr(), 1, work.data_ptr(), work.numel() * 4, device.stream_ptr())
+    assert rel(g.cpu().numpy(), 2 * ref) < 1e-5
+
+
+def _net_and_data(dils, B, H, W, seed=0):
+    from minkasi_amd.learned.precon import init_params, unpack
+    rng = np.random.default_rng(seed)
+    p = init_params(dils, seed=seed, log_alpha=-0.3)
+    ws, la = unpack(p, len(dils))
+    shp = (B, 1, H, W)
+    r = rng.standard_normal(shp).astype(np.float32)
+    e = rng.standard_normal(shp).astype(np.float32)
+    s = rng.uniform(0.2, 1.5, shp).astype(np.float32)
+    s[:, :, :3, :5] = 0.0
+    m = (rng.uniform(size=shp) > 0.3).astype(np.float32)
+    return p, OL.Net([w.astype(np.float64) for w in ws], dils, la), r, e, s, m
+
+
+@pytest.mark.parametrize("dils", [(1, 1, 1, 1), (1, 3, 9, 27)])
+def test_forward_backward_vs_oracle(env, dils):
+    torch, device, mkgpu = env
+    from minkasi_amd.learned.train import Trainer
+    B, H, W = 2, 41, 66
+    p, net, r, e, s, m = _net_and_data(dils, B, H, W)
+    loss, grads, dla = net.loss_and_grad(*(a.astype(np.float64) for a in (r, e, s, m)))
+    tr = Trainer(p, dils, B, H, W)
+    print("forward_backward", dils, flush=True)
+    dev_in = [_t(env, a) for a in (r, e, s, m)]
+    tr.forward_backward(*dev_in)
+    torch.cuda.synchronize()
+    assert abs(tr.loss() - loss) < 1e-5 * loss
+    g = tr.grad.cpu().numpy()
+    ref = np.concatenate([gr.ravel() for gr in grads])
+    assert rel(g[:-1], ref) < 1e-4
+    for (off, co, ci), gr in zip(tr.chain.lay, grads):  # every layer on its own
+        assert rel(g[off:off + co * ci * 9], gr.ravel()) < 2e-4
+    alpha = np.exp(net.log_alpha)
+    assert abs(alpha * tr.dalpha() - dla) < 1e-4 * abs(dla)
+
+
+def test_precon_apply_vs_oracle_and_symmetric(env):
+    torch, device, mkgpu = env
+    from minkasi_amd.learned.precon import LearnedPrecon
+    dils = (1, 3, 9, 27)
+    H, W = 96, 130
+    p, net, r, _, s, _ = _net_and_data(dils, 1, H, W, seed=3)
+    hits = np.where(s[0, 0] > 0, 1.0 / np.maximum(s[0, 0], 1e-30) ** 2, 0.0).astype(np.float32)
+    lp = LearnedPrecon(_t(env, hits), p, dils)
+    s_used = lp.s.cpu().numpy().astype(np.float64)
+    rt = _t(env, r[0, 0])
+    z = torch.empty_like(rt)
+    lp.apply_to(rt, z)
+    ref = net.apply(r[0, 0].astype(np.float64), s_used)
+    assert rel(z.cpu().numpy(), ref) < 1e-5
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((H, W)).astype(np.float32)
+    y = rng.standard_normal((H, W)).astype(np.float32)
+    mx, my = torch.empty_like(rt), torch.empty_like(rt)
+    xd, yd = _t(env, x), _t(env, y)
+    lp.apply_to(xd, mx)
+    lp.apply_to(yd, my)
+    a = float(np.dot(x.ravel().astype(np.float64), my.cpu().numpy().ravel()))
+    b = float(np.dot(mx.cpu().numpy().ravel().astype(np.float64), y.ravel()))
+    assert abs(a - b) < 1e-5 * np.sqrt(abs(float(np.dot(x.ravel(), mx.cpu().numpy().ravel()))
+                                              * float(np.dot(y.ravel(), my.cpu().numpy().ravel()))))
+    assert float(np.dot(x.ravel().astype(np.float64), mx.cpu().numpy().ravel())) > 0
+
+
+def test_adam_matches_numpy(env):
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(4)
+    n = 1000
+    p = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    dal = np.array([0.7, 0.0])
+    pt, gt = _t(env, p), _t(env, g)
+    mt = torch.zeros_like(pt)
+    vt = torch.zeros_like(pt)
+    dt = device.to_device(dal)
+    pr, m, v = p.astype(np.float64), np.zeros(n), np.zeros(n)
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    for step in (1, 2, 3):
+        mkgpu.check("mk_adam_f32", pt.data_ptr(), gt.data_ptr(), mt.data_ptr(), vt.data_ptr(), n, dt.data_ptr(),
+                    lr, b1, b2, eps, step, 0.5, device.stream_ptr())
+        gg = 0.5 * g.astype(np.float64)
+        gg[-1] = 0.5 * np.exp(pr[-1]) * dal[0]
+        m = b1 * m + (1 - b1) * gg
+        v = b2 * v + (1 - b2) * gg * gg
+        pr = pr - lr * (m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps)
+    assert np.max(np.abs(pt.cpu().numpy() - pr)) < 1e-5
+
+
+def test_energy_forward_backward_vs_oracle(env):
+    torch, device, mkgpu = env
+    from minkasi_amd.learned.train import Trainer
+    dils = (1, 2, 1, 3)
+    H, W = 14, 11
+    n = H * W
+    p, net, _, _, s, _ = _net_and_data(dils, 1, H, W, seed=5)
+    rng = np.random.default_rng(8)
+    Q = rng.standard_normal((n, n))
+    A = Q @ Q.T / n + 0.05 * np.eye(n)
+    e = rng.standard_normal((H, W))
+    r = (A @ e.ravel()).reshape(H, W)
+    s2 = s[0, 0].astype(np.float64)
+    loss, grads, dla = net.energy_loss_and_grad(r, e, s2, A)
+    At = device.to_device(A.astype(np.float32))
+
+    def apply_A(z, out):  # dense test operator
+        out.copy_((At @ z.reshape(-1)).reshape(H, W))
+
+    tr = Trainer(p, dils, 1, H, W)
+    ins = [_t(env, a) for a in (r, e, s2)]
+    print("energy_forward_backward", flush=True)
+    tr.energy_forward_backward(ins[0], ins[1], ins[2], apply_A)
+    torch.cuda.synchronize()
+    assert abs(tr.loss() - loss) < 1e-4 * loss
+    g = tr.grad.cpu().numpy()
+    ref = np.concatenate([gr.ravel() for gr in grads])
+    assert rel(g[:-1], ref) < 1e-4
+    assert abs(np.exp(net.log_alpha) * tr.dalpha() - dla) < 1e-4 * abs(dla)
