@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3: setup + steps only")
+    ap.add_argument("--weights", default=os.path.join(HERE, "minkasi_amd", "learned", "weights_c4.npz"),
+                    help="trained learned-preconditioner weights")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -106,8 +110,45 @@ def cpu_baseline(P, budget_s=20.0):
     }
 
 
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_traffic_pcg_64tod_2048.txt")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated), or None."""
+    kernels = {"filter": ("k1_kernel<float, float, 0>", "k2_kernel<float, float, 0>", "k3_kernel<float, float, 0>"),
+               "map2tod": ("map2tod_kernel<float>",), "tod2map": ("t2m_items_kernel<float, float>",
+                                                               "t2m_tiles_kernel<float, float>"),
+               "rot_v": ("rot_bf16_kernel<6>",), "rot_vT": ("rot_bf16_kernel<6>",)}.get(stage)
+    if not kernels or not os.path.exists(TRAFFIC_FILE):
+        return None
+    found = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("bytes_per_dispatch"):
+                _, name, val = line.rstrip("\n").split("\t")
+                found[name] = float(val)
+    if not all(k in found for k in kernels):
+        return None
+    return int(sum(found[k] for k in kernels))
+
+
+def copy_probe(nbytes):
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": int(n * 4), "launches": 3}))
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the batched solver holds every TOD on one device "
               "(multi-GPU sharding over TODs is described in DESIGN.md but not built).", file=sys.stderr)
@@ -154,7 +195,7 @@ def main():
     dom = max(st, key=st.get)
     achieved = sb[dom] / (st[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
             "algorithmic_bytes": int(sb[dom]), "stage_ms": {k: round(v, 3) for k, v in st.items()},
             "stage_frac": {k: round(sb[k] / (st[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k in st}}
     log("stage ms:", json.dumps(roof["stage_ms"]), "frac:", json.dumps(roof["stage_frac"]))
@@ -177,27 +218,28 @@ def main():
         print(json.dumps(result))
         return
 
-    # ---- time to tolerance (recursive residual), Jacobi
+    # ---- time to tolerance (recursive residual): Jacobi, then the learned preconditioner
     if not args.no_solve:
+        from minkasi_amd.workloads import map_distance, solve_to_tol
         res = {}
-        solver2 = DevicePCG(op, b, P.precon)
-        x2 = torch.zeros_like(b)
-        torch.cuda.synchronize()
-        t0 = time.time()
-        solver2.init(x2)
-        it = 0
-        rel = 1.0
-        while it < args.maxiter:
-            solver2.step()
-            it += 1
-            rel = solver2.rel_res()
-            if rel < args.tol:
-                break
-        torch.cuda.synchronize()
-        tj = time.time() - t0
-        res["jacobi"] = {"iters": it, "seconds": round(tj, 3), "rel_res": rel, "reached": bool(rel < args.tol),
-                         "true_rel_res": solver2.true_rel_res()}
+        jac = solve_to_tol(P, None, args.tol, args.maxiter)
+        res["jacobi"] = {k: (round(v, 3) if k == "seconds" else v) for k, v in jac.items()
+                         if k not in ("x", "history")}
         log("Jacobi to tol:", json.dumps(res["jacobi"]))
+        if not args.no_learned and os.path.exists(args.weights):
+            from minkasi_amd.learned.precon import LearnedPrecon, load
+            params, arch, spec, meta = load(args.weights)
+            lp = LearnedPrecon(P.hits, params, arch, spec)
+            lrn = solve_to_tol(P, lp, args.tol, args.maxiter)
+            res["learned"] = {k: (round(v, 3) if k == "seconds" else v) for k, v in lrn.items()
+                              if k not in ("x", "history")}
+            res["learned"].update(arch=arch, spec=list(spec), weights=os.path.relpath(args.weights, HERE))
+            res["speedup_wall"] = round(jac["seconds"] / lrn["seconds"], 3)
+            res["iteration_ratio"] = round(jac["iters"] / max(lrn["iters"], 1), 3)
+            res["map_distance"] = map_distance(P, lrn["x"], jac["x"])
+            log("learned to tol:", json.dumps(res["learned"]), "speedup %.2fx" % res["speedup_wall"])
+        elif not args.no_learned:
+            res["learned"] = {"error": "weights file %s missing" % args.weights}
         result["time_to_tol"] = res
     if not args.no_cpu:
         try:

@@ -288,6 +288,13 @@ int mk_conv3x3_wgrad_f32(const float *a, int32_t Co, const float *x,
                          int32_t Ci, int32_t B, int32_t H, int32_t W,
                          int32_t dil, float *g, int32_t accumulate,
                          void *work, int64_t work_bytes, void *stream);
+/* Box restriction by f with stride f: y[b][I][J] = f^-2 * sum of the f x f
+ * fine pixels of block (I, J) (zero beyond H, W); y is B x ceil(H/f) x
+ * ceil(W/f).  prolong is its adjoint: x[b][i][j] (+)= f^-2 y[b][i/f][j/f]. */
+int mk_restrict_box_f32(const float *x, int32_t B, int32_t H, int32_t W,
+                        int32_t f, float *y, void *stream);
+int mk_prolong_box_f32(const float *y, int32_t B, int32_t H, int32_t W,
+                       int32_t f, float *x, int32_t accumulate, void *stream);
 /* z = s * (exp(log_alpha[0]) * u + g); g may be NULL (Jacobi-like). */
 int mk_precon_finish_f32(float *z, const float *s, const float *log_alpha,
                          const float *u, const float *g, int64_t n,

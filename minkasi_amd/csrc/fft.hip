@@ -126,6 +126,8 @@ __device__ __forceinline__ void dft8(cx<T>* v) {
 // One radix-R Stockham stage over nrows padded rows of length 2^l2len.
 // Every thread loads all of its butterflies' inputs, the workgroup
 // synchronises, then the outputs are written in place.
+// tw: this stage's twiddles, tw[(r-1)*Ns + k] = exp(-2 pi i k r / (R Ns)),
+// so the lanes of a wave (consecutive k) read consecutive words.
 template <typename T, bool INV, int LR>
 __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* tw, int l2len, int nrows, int rs, int Ns,
                                           int lns) {
@@ -142,12 +144,13 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* tw, int l2len
       const cx<T>* base = buf + row * rs;
 #pragma unroll
       for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * q)];
-      int st = k << (l2len - lns - LR);  // (len / (R Ns)) * k
+      if (lns > 0) {
 #pragma unroll
-      for (int r = 1; r < R; r++) {
-        cx<T> w = tw[st * r];
-        if (INV) w = cconj(w);
-        v[u][r] = cmul(v[u][r], w);
+        for (int r = 1; r < R; r++) {
+          cx<T> w = tw[(r - 1) * Ns + k];
+          if (INV) w = cconj(w);
+          v[u][r] = cmul(v[u][r], w);
+        }
       }
     }
   }
@@ -169,31 +172,48 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, const cx<T>* tw, int l2len
   __syncthreads();
 }
 
+// Radix of the stage that follows `rem` remaining levels: 8 while three or
+// more levels remain, then 4 or 2.
+__host__ __device__ __forceinline__ int stage_lr(int rem) { return rem >= 3 ? 3 : rem; }
+
 // In-place FFT of nrows padded rows (row stride rs) of length 2^l2len,
-// nrows * 2^l2len <= kMaxWG; tw[i] = exp(-2 pi i i/len) (unpadded).
+// nrows * 2^l2len <= kMaxWG; tw = stage-ordered table (fill_stage_table).
 template <typename T, bool INV>
 __device__ void lds_fft(cx<T>* buf, const cx<T>* tw, int l2len, int nrows, int rs) {
-  int Ns = 1, lns = 0, rem = l2len;
+  int Ns = 1, lns = 0, rem = l2len, off = 0;
   while (rem > 0) {
-    if (rem >= 3) {
-      fft_stage<T, INV, 3>(buf, tw, l2len, nrows, rs, Ns, lns);
-      Ns <<= 3; lns += 3; rem -= 3;
-    } else if (rem == 2) {
-      fft_stage<T, INV, 2>(buf, tw, l2len, nrows, rs, Ns, lns);
-      Ns <<= 2; lns += 2; rem -= 2;
-    } else {
-      fft_stage<T, INV, 1>(buf, tw, l2len, nrows, rs, Ns, lns);
-      Ns <<= 1; lns += 1; rem -= 1;
-    }
+    const int lr = stage_lr(rem);
+    if (lr == 3) fft_stage<T, INV, 3>(buf, tw + off, l2len, nrows, rs, Ns, lns);
+    else if (lr == 2) fft_stage<T, INV, 2>(buf, tw + off, l2len, nrows, rs, Ns, lns);
+    else fft_stage<T, INV, 1>(buf, tw + off, l2len, nrows, rs, Ns, lns);
+    off += ((1 << lr) - 1) * Ns;
+    Ns <<= lr; lns += lr; rem -= lr;
   }
 }
 
+// Twiddles of every stage of a length-2^l2len FFT, stage after stage:
+// entry (r-1)*Ns + k of a stage is exp(-2 pi i k r / (R Ns)); 2^l2len - 1
+// entries in all.
 template <typename T>
-__device__ void fill_table(cx<T>* tab, int n, int64_t den2) {
-  // tab[i] = exp(-2 pi i i / den2) for i < n (den2 a power of two)
-  const T inv = T(1) / (T)den2;
-  for (int i = threadIdx.x; i < n; i += kThreads) tab[i] = expmi2<T>(2 * (int64_t)i, inv);
+__device__ void fill_stage_table(cx<T>* tab, int l2len) {
+  const T inv = T(1) / (T)(1 << l2len);
+  int Ns = 1, rem = l2len, off = 0;
+  while (rem > 0) {
+    const int lr = stage_lr(rem), R = 1 << lr;
+    const int cnt = (R - 1) * Ns;
+    const int mul = 1 << (l2len - lr - (l2len - rem));  // len / (R Ns)
+    for (int e = threadIdx.x; e < cnt; e += kThreads) {
+      int r = e / Ns + 1, k = e - (r - 1) * Ns;
+      tab[off + e] = expmi2<T>(2 * (int64_t)k * r * mul, inv);
+    }
+    off += cnt;
+    Ns <<= lr; rem -= lr;
+  }
 }
+
+// exp(-2 pi i a / P) for 0 <= a < P, P = 2^lP <= 2^23.
+template <typename T>
+__device__ __forceinline__ cx<T> twiddle_P(int64_t a, T invP) { return expmi2<T>(2 * a, invP); }
 
 // Segment: `rows` rows sharing one length n and one plan.
 struct Seg {
@@ -224,7 +244,7 @@ __device__ inline int find_seg(const Seg* segs, int nseg, int64_t g, int which) 
 // LDS: cw padded rows of length P1 (one per matrix column j2), then the P1
 // twiddle table and the P2 table exp(-2 pi i j/P) for the four-step twiddle.
 template <typename T, typename TIn, int MODE>
-__global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k1_kernel(const Seg* __restrict__ segs, int nseg,
                                                       const TIn* __restrict__ in, cx<T>* __restrict__ tbuf) {
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 1);
@@ -237,12 +257,10 @@ __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ se
   const int64_t P = (int64_t)P1 * P2;
   const int64_t m = sg.n - 1, N = 2 * m;
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [cw][rs]
-  cx<T>* tw = buf + cw * rs;                          // [P1]
-  cx<T>* tb2 = tw + P1;                               // [P2]
+  cx<T>* tw = buf + cw * rs;                          // [P1] stage table
   const TIn* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * cw;
-  fill_table<T>(tw, P1, P1);
-  fill_table<T>(tb2, P2, P);
+  fill_stage_table<T>(tw, l1);
   constexpr int UNR = 8;  // loads in flight per thread
   if (MODE == kFilter || MODE == kFSpec) {
     // real input packed two per complex point: a_(2j), a_(2j+1)
@@ -303,11 +321,12 @@ __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ se
   __syncthreads();
   lds_fft<T, false>(buf, tw, l1, cw, rs);
   cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  const T invP = T(1) / (T)P;
   for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
     int k1 = idx >> lcw, c = idx & (cw - 1);
     int64_t j2 = j2_0 + c;
-    int64_t a = (j2 * k1) & (P - 1);  // w_P^a = w_P1^(a / P2) * w_P^(a % P2)
-    cx<T> v = cmul(buf[c * rs + pad8(k1)], cmul(tw[a >> l2], tb2[a & (P2 - 1)]));
+    int64_t a = (j2 * k1) & (P - 1);
+    cx<T> v = cmul(buf[c * rs + pad8(k1)], twiddle_P<T>(a, invP));
     trow[(int64_t)k1 * P2 + j2] = v;
   }
 }
@@ -321,7 +340,7 @@ __global__ void __launch_bounds__(kThreads) k1_kernel(const Seg* __restrict__ se
 // TS: storage type of real spectra (the FP32 filter keeps FP32 spectra that
 // were computed in FP64).
 template <typename T, typename TS, int MODE>
-__global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(const Seg* __restrict__ segs, int nseg,
                                                       cx<T>* __restrict__ tbuf) {
   extern __shared__ unsigned char smem_raw[];
   __shared__ int slot_k1[16], slot_partner[16];
@@ -354,31 +373,40 @@ __global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ se
     nslot_sh = ns;
   }
   cx<T>* X = reinterpret_cast<cx<T>*>(smem_raw);             // [nslot][rs]
-  cx<T>* tw = X + (real_mode ? 2 * sg.r2 : sg.r2) * rs;      // [P2]  exp(-2 pi i j/P2)
-  cx<T>* tb1 = tw + P2;                                       // [P1]  exp(-2 pi i j/P1)
-  cx<T>* tb2 = tb1 + P1;                                      // [P2]  exp(-2 pi i j/P)
-  fill_table<T>(tw, P2, P2);
-  fill_table<T>(tb1, P1, P1);
-  fill_table<T>(tb2, P2, P);
+  cx<T>* tw = X + (real_mode ? 2 * sg.r2 : sg.r2) * rs;      // [P2] stage table
+  // filter: this workgroup's spectrum rows, fetched with the T rows so their
+  // latency overlaps the forward FFT ([nslot][P2] + B_P)
+  TS* S = reinterpret_cast<TS*>(tw + P2);
+  fill_stage_table<T>(tw, l2);
   __syncthreads();
   const int nslot = nslot_sh;
   cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   {
     constexpr int UNR = 8;
     const int total = nslot * P2;
+    const TS* sbase = reinterpret_cast<const TS*>(sg.s_ptr) + (int64_t)row * sg.s_stride;
     for (int base = threadIdx.x; base < total; base += kThreads * UNR) {
       cx<T> v[UNR];
+      TS b[UNR];
 #pragma unroll
       for (int u = 0; u < UNR; u++) {
         int idx = base + u * kThreads;
-        if (idx < total) v[u] = trow[(int64_t)slot_k1[idx >> l2] * P2 + (idx & (P2 - 1))];
+        if (idx < total) {
+          int64_t off = (int64_t)slot_k1[idx >> l2] * P2 + (idx & (P2 - 1));
+          v[u] = trow[off];
+          if (MODE == kFilter) b[u] = sbase[off];
+        }
       }
 #pragma unroll
       for (int u = 0; u < UNR; u++) {
         int idx = base + u * kThreads;
-        if (idx < total) X[(idx >> l2) * rs + pad8(idx & (P2 - 1))] = v[u];
+        if (idx < total) {
+          X[(idx >> l2) * rs + pad8(idx & (P2 - 1))] = v[u];
+          if (MODE == kFilter) S[idx] = b[u];
+        }
       }
     }
+    if (MODE == kFilter && threadIdx.x == 0) S[total] = sbase[P];
   }
   __syncthreads();
   lds_fft<T, false>(X, tw, l2, nslot, rs);
@@ -419,12 +447,12 @@ __global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ se
         if (k == 0) srow[P] = (TS)(E.re - O.re);                       // A_P
         continue;
       }
-      T b = (T)srow[(int64_t)kk1 * P2 + k2];
-      T bq = (T)srow[(int64_t)pk1 * P2 + pk2];
+      T b = (T)S[idx];
+      T bq = (T)S[pidx];
       cx<T> Y = cscale(A, b), Yq = cscale(Aq, bq);
       if (k == 0) {
         // self pair: the inverse partner of Y_0 is Y_P (real)
-        T yP = (E.re - O.re) * (T)srow[P];
+        T yP = (E.re - O.re) * (T)S[nslot * P2];
         cx<T> Ep = {T(0.5) * (Y.re + yP), T(0.5) * Y.im};
         cx<T> Op = {T(0.5) * (Y.re - yP), T(0.5) * Y.im};
         at(idx) = {Ep.re - Op.im, Ep.im + Op.re};
@@ -443,18 +471,18 @@ __global__ void __launch_bounds__(kThreads) k2_kernel(const Seg* __restrict__ se
     __syncthreads();
   }
   lds_fft<T, true>(X, tw, l2, nslot, rs);
+  const T invP2 = T(1) / (T)P;
   for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads) {
     int sl = idx >> l2, j2 = idx & (P2 - 1);
     int kk1 = slot_k1[sl];
-    int64_t a = ((int64_t)j2 * kk1) & (P - 1);  // conj(w_P^a) from two table entries
-    cx<T> tw_a = cmul(tb1[a >> l2], tb2[a & (P2 - 1)]);
-    trow[(int64_t)kk1 * P2 + j2] = cmul(at(idx), cconj(tw_a));
+    int64_t a = ((int64_t)j2 * kk1) & (P - 1);
+    trow[(int64_t)kk1 * P2 + j2] = cmul(at(idx), cconj(twiddle_P<T>(a, invP2)));
   }
 }
 
 // ---- K3: inverse column FFTs + output -------------------------------------
 template <typename T, typename TOut, int MODE>
-__global__ void __launch_bounds__(kThreads) k3_kernel(const Seg* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k3_kernel(const Seg* __restrict__ segs, int nseg,
                                                       const cx<T>* __restrict__ tbuf, TOut* __restrict__ out) {
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 3);
@@ -467,10 +495,10 @@ __global__ void __launch_bounds__(kThreads) k3_kernel(const Seg* __restrict__ se
   const int64_t P = (int64_t)P1 * P2;
   const int64_t m = sg.n - 1, N = 2 * m;
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [cw][rs]
-  cx<T>* tw = buf + cw * rs;                          // [P1]
+  cx<T>* tw = buf + cw * rs;                          // [P1] stage table
   const cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   const int j2_0 = jb * cw;
-  fill_table<T>(tw, P1, P1);
+  fill_stage_table<T>(tw, l1);
   {
     constexpr int UNR = 8;
     const int total = P1 * cw;
@@ -538,27 +566,26 @@ struct Geometry {
   int64_t P;
 };
 
-// P = 2^q complex points split as P1 = 2^floor(q/2) <= P2.
+// P = 2^q complex points split as P1 * P2 with the row length P2 as long as
+// K2 holds (2^11 points, a real-mode row and its partner fill one
+// workgroup), so the columns are short and K1/K3 can take many adjacent
+// columns at once: their global accesses are runs of cw complex points.
+// MK_FFT_L2MAX (power-of-two exponent, 2..11) lowers the row-length cap.
 Geometry geometry(int64_t P_min, int elem_bytes) {
   int q = std::max(2, log2_ceil(P_min));
+  static int l2max = [] {
+    const char* e = getenv("MK_FFT_L2MAX");
+    int v = e ? atoi(e) : 11;
+    return std::min(11, std::max(2, v));
+  }();
   Geometry g;
-  g.l1 = q / 2;
-  g.l2 = q - g.l1;
+  g.l2 = std::min(l2max, q - 1);
+  g.l1 = q - g.l2;
   g.P = int64_t(1) << q;
   int P1 = 1 << g.l1, P2 = 1 << g.l2;
-  // columns per K1/K3 workgroup: MK_FFT_CW (a power of two, set by the
-  // autotuner) caps the default of 16
-  static int cw_cap = [] {
-    const char* e = getenv("MK_FFT_CW");
-    int v = e ? atoi(e) : 16;
-    int c = 1;
-    while (c * 2 <= v && c < 64) c *= 2;
-    return c;
-  }();
-  int cw = std::min(cw_cap, std::max(1, kMaxWG / P1));
-  cw = std::min(cw, P2);
-  // keep the K1/K3 tile plus twiddle tables within the LDS budget
-  while (cw > 1 && ((int64_t)cw * row_stride(P1) + P1 + P2) * 2 * elem_bytes > kMaxLds) cw >>= 1;
+  int cw = std::min(std::max(1, kMaxWG / P1), P2);
+  // keep the K1/K3 tile plus the twiddle table within the LDS budget
+  while (cw > 1 && ((int64_t)cw * row_stride(P1) + P1) * 2 * elem_bytes > kMaxLds) cw >>= 1;
   g.cw = cw;
   return g;
 }
@@ -598,20 +625,24 @@ void allow_lds_all() {
 }
 
 size_t lds_k13(const Geometry& g, int eb) {
-  return (size_t)(g.cw * row_stride(1 << g.l1) + (1 << g.l1) + (1 << g.l2)) * 2 * eb;
+  return (size_t)(g.cw * row_stride(1 << g.l1) + (1 << g.l1)) * 2 * eb;
+}
+// k1 values per K2 workgroup: slots (2 per k1 for real modes) * P2 <= kMaxWG
+// LDS of K2: slot rows, the stage table and (real modes) the slots' spectrum
+// values, one real each plus B_P.
+size_t k2_bytes(int r, int P2, int eb, bool real) {
+  int ns = real ? 2 * r : r;
+  return (size_t)(ns * row_stride(P2) + P2) * 2 * eb + (real ? (size_t)(ns * P2 + 1) * eb : 0) + 64;
 }
 // k1 values per K2 workgroup: slots (2 per k1 for real modes) * P2 <= kMaxWG
 int k2_r2(const Geometry& g, int eb, bool real) {
   int P2 = 1 << g.l2, P1 = 1 << g.l1;
-  int r = 4;
-  while (r > 1 && ((real ? 2 : 1) * r * P2 > kMaxWG ||
-                   (size_t)((real ? 2 * r : r) * row_stride(P2) + 2 * P2 + P1) * 2 * eb > (size_t)kMaxLds))
-    r >>= 1;
+  int r = 8;
+  while (r > 1 && ((real ? 2 : 1) * r * P2 > kMaxWG || k2_bytes(r, P2, eb, real) > (size_t)kMaxLds)) r >>= 1;
   return std::min(r, real ? P1 / 2 + 1 : P1);
 }
 size_t lds_k2(const Geometry& g, int eb, bool real = true) {
-  int r = k2_r2(g, eb, real);
-  return (size_t)((real ? 2 * r : r) * row_stride(1 << g.l2) + 2 * (1 << g.l2) + (1 << g.l1)) * 2 * eb + 64;
+  return k2_bytes(k2_r2(g, eb, real), 1 << g.l2, eb, real);
 }
 
 }  // namespace

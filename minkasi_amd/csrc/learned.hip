@@ -284,6 +284,42 @@ __global__ void __launch_bounds__(256) precon_aloss_stage1(const float* __restri
   }
 }
 
+// Box restriction by f (stride f): y[b,I,J] = f^-2 sum_{a,c<f} x[b, fI+a, fJ+c],
+// zero beyond the fine grid; the coarse grid is ceil(H/f) x ceil(W/f).
+__global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__ x, int H, int W, int f, int Hc,
+                                                       int Wc, int64_t total, float* __restrict__ y) {
+  const float inv = 1.f / (float)(f * f);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = k / ((int64_t)Hc * Wc);
+    int rem = (int)(k - b * Hc * Wc);
+    int I = rem / Wc, J = rem - I * Wc;
+    const float* xb = x + b * (int64_t)H * W;
+    float s = 0.f;
+    for (int a = 0; a < f; a++) {
+      int i = I * f + a;
+      if (i >= H) break;
+      for (int c = 0; c < f; c++) {
+        int j = J * f + c;
+        if (j < W) s += xb[(int64_t)i * W + j];
+      }
+    }
+    y[k] = s * inv;
+  }
+}
+
+// Adjoint of restrict_kernel: x[b,i,j] (+)= f^-2 y[b, i/f, j/f].
+__global__ void __launch_bounds__(256) prolong_kernel(const float* __restrict__ y, int Hc, int Wc, int f, int H, int W,
+                                                      int64_t total, float* __restrict__ x, int accumulate) {
+  const float inv = 1.f / (float)(f * f);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = k / ((int64_t)H * W);
+    int rem = (int)(k - b * H * W);
+    int i = rem / W, j = rem - i * W;
+    float v = y[b * (int64_t)Hc * Wc + (int64_t)(i / f) * Wc + j / f] * inv;
+    x[k] = accumulate ? x[k] + v : v;
+  }
+}
+
 // Adam on a flat FP32 parameter vector.  Element n-1 is log(alpha); its
 // gradient is alpha * dL/dalpha, taken from dalpha (FP64, on the device).
 __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -431,4 +467,24 @@ extern "C" int mk_precon_aloss_f32(const float* z, const float* az, const float*
   if (rc) return rc;
   precon_loss_stage2<<<1, 256, 0, st>>>((const double*)work, sc);
   return mk::check_launch("precon_loss_stage2");
+}
+
+extern "C" int mk_restrict_box_f32(const float* x, int32_t B, int32_t H, int32_t W, int32_t f, float* y, void* stream) {
+  MK_REQUIRE(x && y && x != y, MK_ERR_ARG, "restrict: bad pointers");
+  MK_REQUIRE(B > 0 && H > 0 && W > 0 && f >= 1 && f <= 64, MK_ERR_ARG, "restrict: bad shape");
+  int Hc = (int)mk::cdiv(H, f), Wc = (int)mk::cdiv(W, f);
+  int64_t total = (int64_t)B * Hc * Wc;
+  restrict_kernel<<<mk::grid_for(total, 256), 256, 0, mk::as_stream(stream)>>>(x, H, W, f, Hc, Wc, total, y);
+  return mk::check_launch("restrict_kernel");
+}
+
+extern "C" int mk_prolong_box_f32(const float* y, int32_t B, int32_t H, int32_t W, int32_t f, float* x,
+                                  int32_t accumulate, void* stream) {
+  MK_REQUIRE(x && y && x != y, MK_ERR_ARG, "prolong: bad pointers");
+  MK_REQUIRE(B > 0 && H > 0 && W > 0 && f >= 1 && f <= 64, MK_ERR_ARG, "prolong: bad shape");
+  int Hc = (int)mk::cdiv(H, f), Wc = (int)mk::cdiv(W, f);
+  int64_t total = (int64_t)B * H * W;
+  prolong_kernel<<<mk::grid_for(total, 256), 256, 0, mk::as_stream(stream)>>>(y, Hc, Wc, f, H, W, total, x,
+                                                                               accumulate);
+  return mk::check_launch("prolong_kernel");
 }

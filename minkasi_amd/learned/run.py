@@ -1,13 +1,15 @@
 """Harvest, train and evaluate the learned preconditioner on one GPU.
 
-    python -m minkasi_amd.learned.run --seeds 32 --ntod 64 --pairs 5000 \
+    python -m minkasi_amd.learned.run --seeds 32 --ntod 64 \
         --out minkasi_amd/learned/weights_c4.npz [--eval]
 
 Training problems use the 64-TOD, 2048^2 distribution with TOD seeds and
 shapes disjoint from the benchmark problem (seed_base = 10000 + 1000 * seed,
-shape seed 20000 + seed).  --eval then solves the benchmark problem
-(seed_base 0) to ||r||/||b|| < tol with Jacobi and with the trained network
-and prints one JSON line.
+shape seed 20000 + seed).  Each is solved with plain Jacobi PCG and its
+(residual, error) pairs feed the energy loss (default; the problem's own
+operator supplies A z) or the crop loss.  --eval then solves the benchmark
+problem (seed_base 0) to ||r||/||b|| < tol with Jacobi and with the trained
+network and prints one JSON line.
 """
 import argparse
 import json
@@ -24,19 +26,25 @@ def log(*a):
 def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--arch", default="levels", choices=["levels", "chain"])
+    ap.add_argument("--spec", default="1,4,16,64", help="level factors (levels) or dilations (chain)")
+    ap.add_argument("--loss", default="energy", choices=["energy", "crop"])
     ap.add_argument("--seeds", type=int, default=32)
     ap.add_argument("--ntod", type=int, default=64, help="TODs per training problem")
-    ap.add_argument("--pairs", type=int, default=5000)
+    ap.add_argument("--steps-per-seed", type=int, default=60, help="energy loss: Adam steps per problem")
+    ap.add_argument("--pairs", type=int, default=5000, help="crop loss: crops in the dataset")
     ap.add_argument("--crop", type=int, default=512)
-    ap.add_argument("--dils", default="1,3,9,27")
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--init-frac", type=float, default=0.3, help="initial K^T K gain relative to alpha")
+    ap.add_argument("--steps", type=int, default=1500, help="crop loss: Adam steps")
+    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--init-frac", type=float, default=0.01, help="initial K^T K gain relative to alpha")
     ap.add_argument("--max-iter", type=int, default=150)
     ap.add_argument("--tol-final", type=float, default=5e-8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--init", default=None, help="start from these weights (.npz from save or .npy params)")
+    ap.add_argument("--calibrate", default="", help="K^T K scale factors tried on a held-out problem, e.g. "
+                    "0.25,0.5,1,2 (empty: none)")
     ap.add_argument("--eval", action="store_true")
     ap.add_argument("--eval-ntod", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -98,21 +106,36 @@ def build_dataset(config, seeds, ntod, pairs, crop, margin, rng, max_iter, tol_f
     return Dataset(*out)
 
 
-def train(ds, dils, steps, lr, batch, seed=0, init_frac=0.3, log_every=100):
-    """Adam on the training crops; returns (params numpy, history, meta).
-    The loss ignores pixels within the reach of K^T K (2 * sum(dils)) of a
-    crop's border, where the crop lacks inputs the full map has."""
-    from .precon import init_params, param_layout
-    from .train import Trainer
-    ds.set_margin(2 * sum(dils))
-    alpha0, base_loss = ds.alpha_fit()
-    lay, _ = param_layout(len(dils))
-    p0 = init_params(dils, seed=seed, scale=1.0, log_alpha=np.log(alpha0))
-    gain = (init_frac * alpha0) ** (1.0 / (2 * len(dils)))
+def reach(arch, spec):
+    """Pixels within which K^T K u depends on u."""
+    if arch == "chain":
+        return 2 * sum(spec)
+    return 2 * max(spec) + max(spec)  # 3x3 taps of the coarsest level, plus its block
+
+
+def init_scaled(arch, spec, alpha0, init_frac, seed):
+    """Random weights with K^T K about init_frac * alpha in gain."""
+    from .nets import param_layout
+    from .precon import init_params
+    lay, _ = param_layout(arch, len(spec))
+    p0 = init_params(arch, spec, seed=seed, scale=1.0, log_alpha=np.log(alpha0))
+    depth = len(spec) if arch == "chain" else 1
+    gain = (init_frac * alpha0) ** (1.0 / (2 * depth))
     for off, co, ci in lay:
-        p0[off:off + co * ci * 9] *= gain / np.sqrt(9.0 * ci)
+        p0[off:off + co * ci * 9] *= gain / np.sqrt(9.0 * ci * (1 if arch == "chain" else co))
+    return p0
+
+
+def train(ds, arch, spec, steps, lr, batch, seed=0, init_frac=0.3, log_every=100):
+    """Crop-loss Adam on the training crops; returns (params numpy, history,
+    meta).  The loss ignores pixels within the reach of K^T K of a crop's
+    border, where the crop lacks inputs the full map has."""
+    from .train import Trainer
+    ds.set_margin(reach(arch, spec))
+    alpha0, base_loss = ds.alpha_fit()
+    p0 = init_scaled(arch, spec, alpha0, init_frac, seed)
     B, C = batch, ds.crop
-    tr = Trainer(p0, dils, B, C, C, lr=lr)
+    tr = Trainer(p0, arch, spec, B, C, C, lr=lr)
     R, E, S, M = ds.tr
     bufs = [torch.empty((B, 1, C, C), dtype=torch.float32, device=R.device) for _ in range(4)]
     g = torch.Generator(device="cpu")
@@ -133,9 +156,10 @@ def train(ds, dils, steps, lr, batch, seed=0, init_frac=0.3, log_every=100):
                 it, l, base_loss, float(np.exp(tr.params[-1].item())), 1000 * (time.time() - t1) / (it + 1)))
     torch.cuda.synchronize()
     params = tr.params.cpu().numpy()
-    vloss = val_loss(ds, params, dils, B)
-    meta = dict(dils=list(dils), crops=ds.N, crop=C, steps=steps, batch=B, lr=lr, init_frac=init_frac,
-                alpha_fit=alpha0, alpha_fit_loss=base_loss, val_loss=vloss, train_s=time.time() - t1)
+    vloss = val_loss(ds, params, arch, spec)
+    meta = dict(loss="crop", arch=arch, spec=list(spec), crops=ds.N, crop=C, steps=steps, batch=B, lr=lr,
+                init_frac=init_frac, alpha_fit=alpha0, alpha_fit_loss=base_loss, val_loss=vloss,
+                train_s=time.time() - t1)
     log("  trained: validation loss/crop %.4f (alpha-only fit on training crops %.4f)" % (vloss, base_loss))
     return params, hist, meta
 
@@ -164,16 +188,14 @@ def energy_pairs(P, max_iter, tol_final, min_ratio=10.0):
     return pairs, rel
 
 
-def train_energy(config, seeds, ntod, steps_per_seed, dils, lr, seed=0, init_frac=0.01, max_iter=150,
+def train_energy(config, seeds, ntod, steps_per_seed, arch, spec, lr, seed=0, init_frac=0.01, max_iter=150,
                  tol_final=5e-8, shapes_seed0=20000, seed_base0=10000, log_every=25, warmup=50, init=None):
     """Energy-norm training on harvested full-map pairs, one problem at a
     time (its operator supplies A z).  Returns (params, history, meta)."""
     from ..synth import tod_shapes
     from ..workloads import build_problem
-    from .precon import init_params, param_layout
     from .train import Trainer
     rng = np.random.default_rng(seed)
-    lay, _ = param_layout(len(dils))
     tr = None
     hist = []
     total = seeds * steps_per_seed
@@ -199,11 +221,8 @@ def train_energy(config, seeds, ntod, steps_per_seed, dils, lr, seed=0, init_fra
             if init is not None:
                 p0 = np.array(init, dtype=np.float32, copy=True)
             else:
-                p0 = init_params(dils, seed=seed, scale=1.0, log_alpha=np.log(alpha0))
-                gain = (init_frac * alpha0) ** (1.0 / (2 * len(dils)))
-                for off, co, ci in lay:
-                    p0[off:off + co * ci * 9] *= gain / np.sqrt(9.0 * ci)
-            tr = Trainer(p0, dils, 1, hits.shape[0], hits.shape[1], lr=lr)
+                p0 = init_scaled(arch, spec, alpha0, init_frac, seed)
+            tr = Trainer(p0, arch, spec, 1, hits.shape[0], hits.shape[1], lr=lr)
             # energy loss of the pure Jacobi step on the first pairs (the reference level)
             jl = []
             for r_, e_, _ in pairs[:: max(1, len(pairs) // 8)]:
@@ -212,8 +231,7 @@ def train_energy(config, seeds, ntod, steps_per_seed, dils, lr, seed=0, init_fra
                 op.apply(zj, azj, accumulate=False)
                 jl.append(float(((zj - e_).double() * (azj - r_).double()).sum()))
             jac_loss = float(np.mean(jl))
-            tr.z = torch.empty(hits.shape, dtype=torch.float32, device=hits.device)
-            tr.az = torch.empty_like(tr.z)
+            tr.energy_buffers()
             log("  alpha of the Jacobi step: %.4g; its energy loss %.4f" % (alpha0, jac_loss))
         op.capture(tr.z, tr.az, accumulate=False)
 
@@ -239,21 +257,21 @@ def train_energy(config, seeds, ntod, steps_per_seed, dils, lr, seed=0, init_fra
         del P, pairs, op
         torch.cuda.empty_cache()
     params = tr.params.cpu().numpy()
-    meta = dict(loss="energy", dils=list(dils), seeds=seeds, ntod=ntod, pairs=npairs, steps=total, lr=lr,
+    meta = dict(loss="energy", arch=arch, spec=list(spec), seeds=seeds, ntod=ntod, pairs=npairs, steps=total, lr=lr,
                 init_frac=init_frac, train_s=time.time() - t0, jacobi_step_loss=jac_loss,
                 final_loss=float(np.mean([h[1] for h in hist[-8:]])) if hist else None)
     return params, hist, meta
 
 
-def val_loss(ds, params, dils, B):
-    """Loss per crop on the validation crops (forward only)."""
+def val_loss(ds, params, arch, spec):
+    """Loss per crop on the validation crops."""
     from .train import Trainer
     R, E, S, M = ds.val
     n = R.shape[0]
     if n == 0:
         return float("nan")
     C = ds.crop
-    tr = Trainer(params, dils, 1, C, C)
+    tr = Trainer(params, arch, spec, 1, C, C)
     tot = 0.0
     for i in range(n):
         tr.forward_backward(*(t[i:i + 1].reshape(1, 1, C, C).contiguous() for t in (R, E, S, M)))
@@ -261,18 +279,19 @@ def val_loss(ds, params, dils, B):
     return tot / n
 
 
-def scale_ktk(params, dils, c):
-    """Parameters with K^T K scaled by c (every layer by c^(1/(2L))), alpha kept."""
-    from .precon import param_layout
+def scale_ktk(params, arch, spec, c):
+    """Parameters with K^T K scaled by c, alpha kept."""
+    from .nets import param_layout
     p = np.array(params, dtype=np.float32, copy=True)
-    lay, _ = param_layout(len(dils))
-    f = np.float32(c ** (1.0 / (2 * len(dils))))
+    lay, _ = param_layout(arch, len(spec))
+    depth = len(spec) if arch == "chain" else 1
+    f = np.float32(c ** (1.0 / (2 * depth)))
     for off, co, ci in lay:
         p[off:off + co * ci * 9] *= f
     return p
 
 
-def calibrate(P, params, dils, tol, maxiter, factors=(0.25, 0.5, 1.0, 2.0, 4.0)):
+def calibrate(P, params, arch, spec, tol, maxiter, factors=(0.25, 0.5, 1.0, 2.0, 4.0)):
     """Picks the K^T K scale (relative to alpha) that needs the fewest PCG
     iterations to tol on problem P (a held-out training problem): CG is
     invariant to the scale of M, not to the balance of its two terms."""
@@ -281,22 +300,22 @@ def calibrate(P, params, dils, tol, maxiter, factors=(0.25, 0.5, 1.0, 2.0, 4.0))
     best = None
     rows = []
     for c in factors:
-        pc = scale_ktk(params, dils, c)
-        res = solve_to_tol(P, LearnedPrecon(P.hits, pc, dils), tol, maxiter)
+        pc = scale_ktk(params, arch, spec, c)
+        res = solve_to_tol(P, LearnedPrecon(P.hits, pc, arch, spec), tol, maxiter)
         its = res["iters"] if res["reached"] else maxiter + 1
         rows.append((c, its))
         log("  calibrate: K^T K scale %.3g -> %d iterations" % (c, its))
         if best is None or its < best[1]:
             best = (c, its)
-    return scale_ktk(params, dils, best[0]), best[0], rows
+    return scale_ktk(params, arch, spec, best[0]), best[0], rows
 
 
-def evaluate(P, params, dils, tol, maxiter, jac=None):
+def evaluate(P, params, arch, spec, tol, maxiter, jac=None):
     from ..workloads import map_distance, solve_to_tol
     from .precon import LearnedPrecon
     if jac is None:
         jac = solve_to_tol(P, None, tol, maxiter)
-    lp = LearnedPrecon(P.hits, params, dils)
+    lp = LearnedPrecon(P.hits, params, arch, spec)
     lrn = solve_to_tol(P, lp, tol, maxiter)
     dist = map_distance(P, lrn["x"], jac["x"])
     out = dict(learned=dict((k, v) for k, v in lrn.items() if k not in ("x", "history")),
@@ -310,30 +329,55 @@ def evaluate(P, params, dils, tol, maxiter, jac=None):
 def main(argv=None):
     args = parse(argv)
     from ..workloads import build_problem
-    from .precon import save
+    from .precon import load, save
     torch.cuda.set_device(0)
-    dils = tuple(int(d) for d in args.dils.split(","))
-    margin = 2 * sum(dils)
-    rng = np.random.default_rng(args.seed)
-    ds = build_dataset(args.config, args.seeds, args.ntod, args.pairs, args.crop, 0, rng, args.max_iter,
-                       args.tol_final)
-    log("dataset: %d training + %d validation crops of %d^2, loss margin %d" % (
-        ds.N, ds.val[0].shape[0], args.crop, margin))
-    params, hist, meta = train(ds, dils, args.steps, args.lr, args.batch, args.seed, args.init_frac)
-    meta.update(seeds=args.seeds, ntod=args.ntod, history=hist)
-    if args.out:
-        save(args.out, params, dils, meta)
-        log("saved", args.out)
-    del ds
+    arch = args.arch
+    spec = tuple(int(d) for d in args.spec.split(","))
+    init = None
+    if args.init:
+        if args.init.endswith(".npz"):
+            init, a0, s0, _ = load(args.init)
+            if (a0, s0) != (arch, spec):
+                raise ValueError("--init weights are %s%s, not %s%s" % (a0, s0, arch, spec))
+        else:
+            init = np.load(args.init, allow_pickle=False)
+    if args.loss == "energy":
+        params, hist, meta = train_energy(args.config, args.seeds, args.ntod, args.steps_per_seed, arch, spec,
+                                          args.lr, args.seed, args.init_frac, args.max_iter, args.tol_final,
+                                          init=init)
+    else:
+        rng = np.random.default_rng(args.seed)
+        ds = build_dataset(args.config, args.seeds, args.ntod, args.pairs, args.crop, 0, rng, args.max_iter,
+                           args.tol_final)
+        log("dataset: %d training + %d validation crops of %d^2" % (ds.N, ds.val[0].shape[0], args.crop))
+        params, hist, meta = train(ds, arch, spec, args.steps, args.lr, args.batch, args.seed, args.init_frac)
+        del ds
+    meta.update(history=hist)
     torch.cuda.empty_cache()
+    if args.calibrate:
+        from ..synth import tod_shapes
+        factors = tuple(float(c) for c in args.calibrate.split(","))
+        # held out from both the training problems and the benchmark problem
+        Pc = build_problem(args.config, shapes=tod_shapes(args.config, args.ntod, seed=29999),
+                           seed_base=9_000_000)
+        params, cbest, rows = calibrate(Pc, params, arch, spec, args.tol, args.maxiter, factors)
+        meta.update(calibration=dict(factors=factors, iterations=[r[1] for r in rows], chosen=cbest))
+        log("calibrated K^T K scale: %g" % cbest)
+        del Pc
+        torch.cuda.empty_cache()
     res = dict(train=meta)
     if args.eval:
         P = build_problem(args.config, ntod=args.eval_ntod)
-        ev, _ = evaluate(P, params, dils, args.tol, args.maxiter)
+        ev, _ = evaluate(P, params, arch, spec, args.tol, args.maxiter)
         res.update(ev)
+        meta["eval"] = dict(ntod=len(P.shapes), jacobi_iters=ev["jacobi"]["iters"],
+                            learned_iters=ev["learned"]["iters"], speedup=ev["speedup"])
         log("Jacobi: %d its %.2f s; learned: %d its %.2f s; speedup %.2fx; map distance %.2e" % (
             ev["jacobi"]["iters"], ev["jacobi"]["seconds"], ev["learned"]["iters"], ev["learned"]["seconds"],
             ev["speedup"], ev["map_distance"]))
+    if args.out:
+        save(args.out, params, arch, spec, meta)
+        log("saved", args.out)
     print(json.dumps(res, default=float))
 
 

@@ -15,11 +15,16 @@ Training losses for (r, e) pairs: the crop loss L = sum m (M r - e)^2 with
 per-pixel weights m, and the energy loss L = (M r - e)^T A (M r - e) with
 A e = r; parameters = the conv weights and log(alpha).
 
+LevelsNet replaces the chain by one conv per level of box-restricted maps
+(restrict / prolong below): K^T K = sum_l (R^l)^T C_l^T C_l R^l.
+
 Pins (tests/test_oracle_learned.py): conv3x3 against scipy.signal.correlate2d
 with the explicitly dilated kernel; the adjoint identity <conv x, y> =
 <x, conv^T y>; wgrad against brute force with unit weights; loss_and_grad
 and energy_loss_and_grad against central finite differences; M symmetric and positive semidefinite;
-zero weights with alpha = 1 reduce M to the hits-count Jacobi 1/hits.
+zero weights with alpha = 1 reduce M to the hits-count Jacobi 1/hits;
+restrict/prolong adjoint to each other and restrict equal to block means;
+LevelsNet gradients against finite differences and symmetric PSD.
 """
 import numpy as np
 
@@ -146,4 +151,69 @@ class Net:
             grads[l - 1] += wgrad(bgrad, hs[l - 1], self.dils[l - 1])
             if l > 1:
                 bgrad = conv3x3(bgrad, adjoint_weights(self.ws[l - 1]), self.dils[l - 1])
+        return grads, dalpha * alpha
+
+
+def restrict(x, f):
+    """Box restriction by f, stride f: y[b,c,I,J] = f^-2 sum of block (I, J),
+    zero beyond the grid; output ceil(H/f) x ceil(W/f)."""
+    x = np.asarray(x, dtype=np.float64)
+    B, C, H, W = x.shape
+    Hc, Wc = -(-H // f), -(-W // f)
+    pad = np.zeros((B, C, Hc * f, Wc * f))
+    pad[:, :, :H, :W] = x
+    return pad.reshape(B, C, Hc, f, Wc, f).sum(axis=(3, 5)) / (f * f)
+
+
+def prolong(y, f, H, W):
+    """Adjoint of restrict: x[b,c,i,j] = f^-2 y[b,c,i//f,j//f]."""
+    y = np.asarray(y, dtype=np.float64)
+    x = np.repeat(np.repeat(y, f, axis=2), f, axis=3)[:, :, :H, :W]
+    return x / (f * f)
+
+
+class LevelsNet(Net):
+    """K u = [C_0 u; C_1 R^1 u; ...]: one 3x3 conv (Cin = 1) per level, level l
+    on the map restricted by factors[l] (successive box restrictions by
+    factors[l] / factors[l-1]); K^T K = sum_l (R^l)^T C_l^T C_l R^l."""
+
+    def __init__(self, ws, factors, log_alpha=0.0):
+        super().__init__(ws, [1] * len(ws), log_alpha)
+        self.factors = list(factors)
+        self.steps = [1] + [self.factors[i] // self.factors[i - 1] for i in range(1, len(self.factors))]
+
+    def _levels(self, u):
+        us = [u]
+        for st in self.steps[1:]:
+            us.append(restrict(us[-1], st))
+        return us
+
+    def _ktk(self, us):
+        gs = [conv3x3(conv3x3(ul, w, 1), adjoint_weights(w), 1) for ul, w in zip(us, self.ws)]
+        acc = gs[-1]
+        for l in range(len(gs) - 1, 0, -1):
+            H, W = gs[l - 1].shape[-2:]
+            acc = gs[l - 1] + prolong(acc, self.steps[l], H, W)
+        return acc
+
+    def apply(self, r, s):
+        shape = np.shape(r)
+        r = np.asarray(r, dtype=np.float64).reshape((-1, 1) + shape[-2:])
+        s = np.asarray(s, dtype=np.float64).reshape((-1, 1) + shape[-2:])
+        u = s * r
+        z = s * (np.exp(self.log_alpha) * u + self._ktk(self._levels(u)))
+        return z.reshape(shape)
+
+    def _forward(self, r, s):
+        u = s * r
+        us = self._levels(u)
+        return u, us, [self._ktk(us)]
+
+    def _backward(self, q, u, us, gs):
+        alpha = np.exp(self.log_alpha)
+        dalpha = float(np.sum(q * u))
+        qs = self._levels(q)
+        grads = []
+        for ul, ql, w in zip(us, qs, self.ws):
+            grads.append(wgrad(conv3x3(ul, w, 1), ql, 1) + wgrad(conv3x3(ql, w, 1), ul, 1))
         return grads, dalpha * alpha

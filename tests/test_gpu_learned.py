@@ -88,29 +88,34 @@ def test_wgrad(env, co, ci, dil):
     assert rel(g.cpu().numpy(), 2 * ref) < 1e-5
 
 
-def _net_and_data(dils, B, H, W, seed=0):
+def _net_and_data(arch, spec, B, H, W, seed=0):
     from minkasi_amd.learned.precon import init_params, unpack
     rng = np.random.default_rng(seed)
-    p = init_params(dils, seed=seed, log_alpha=-0.3)
-    ws, la = unpack(p, len(dils))
+    p = init_params(arch, spec, seed=seed, log_alpha=-0.3)
+    ws, la = unpack(p, arch, len(spec))
     shp = (B, 1, H, W)
     r = rng.standard_normal(shp).astype(np.float32)
     e = rng.standard_normal(shp).astype(np.float32)
     s = rng.uniform(0.2, 1.5, shp).astype(np.float32)
     s[:, :, :3, :5] = 0.0
     m = (rng.uniform(size=shp) > 0.3).astype(np.float32)
-    return p, OL.Net([w.astype(np.float64) for w in ws], dils, la), r, e, s, m
+    ws64 = [w.astype(np.float64) for w in ws]
+    net = OL.Net(ws64, spec, la) if arch == "chain" else OL.LevelsNet(ws64, spec, la)
+    return p, net, r, e, s, m
 
 
-@pytest.mark.parametrize("dils", [(1, 1, 1, 1), (1, 3, 9, 27)])
-def test_forward_backward_vs_oracle(env, dils):
+ARCHS = [("chain", (1, 1, 1, 1)), ("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64)), ("levels", (1, 3, 6))]
+
+
+@pytest.mark.parametrize("arch,spec", ARCHS)
+def test_forward_backward_vs_oracle(env, arch, spec):
     torch, device, mkgpu = env
     from minkasi_amd.learned.train import Trainer
     B, H, W = 2, 41, 66
-    p, net, r, e, s, m = _net_and_data(dils, B, H, W)
+    p, net, r, e, s, m = _net_and_data(arch, spec, B, H, W)
     loss, grads, dla = net.loss_and_grad(*(a.astype(np.float64) for a in (r, e, s, m)))
-    tr = Trainer(p, dils, B, H, W)
-    print("forward_backward", dils, flush=True)
+    tr = Trainer(p, arch, spec, B, H, W)
+    print("forward_backward", arch, spec, flush=True)
     dev_in = [_t(env, a) for a in (r, e, s, m)]
     tr.forward_backward(*dev_in)
     torch.cuda.synchronize()
@@ -118,20 +123,21 @@ def test_forward_backward_vs_oracle(env, dils):
     g = tr.grad.cpu().numpy()
     ref = np.concatenate([gr.ravel() for gr in grads])
     assert rel(g[:-1], ref) < 1e-4
-    for (off, co, ci), gr in zip(tr.chain.lay, grads):  # every layer on its own
+    for (off, co, ci), gr in zip(tr.net.lay, grads):  # every layer on its own
         assert rel(g[off:off + co * ci * 9], gr.ravel()) < 2e-4
     alpha = np.exp(net.log_alpha)
     assert abs(alpha * tr.dalpha() - dla) < 1e-4 * abs(dla)
 
 
-def test_precon_apply_vs_oracle_and_symmetric(env):
+@pytest.mark.parametrize("arch,spec", [("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64))])
+def test_precon_apply_vs_oracle_and_symmetric(env, arch, spec):
     torch, device, mkgpu = env
     from minkasi_amd.learned.precon import LearnedPrecon
-    dils = (1, 3, 9, 27)
     H, W = 96, 130
-    p, net, r, _, s, _ = _net_and_data(dils, 1, H, W, seed=3)
-    hits = np.where(s[0, 0] > 0, 1.0 / np.maximum(s[0, 0], 1e-30) ** 2, 0.0).astype(np.float32)
-    lp = LearnedPrecon(_t(env, hits), p, dils)
+    p, net, r, _, s, _ = _net_and_data(arch, spec, 1, H, W, seed=3)
+    s00 = s[0, 0].astype(np.float64)
+    hits = np.where(s00 > 0, 1.0 / np.where(s00 > 0, s00, 1.0) ** 2, 0.0).astype(np.float32)
+    lp = LearnedPrecon(_t(env, hits), p, arch, spec)
     s_used = lp.s.cpu().numpy().astype(np.float64)
     rt = _t(env, r[0, 0])
     z = torch.empty_like(rt)
@@ -147,9 +153,25 @@ def test_precon_apply_vs_oracle_and_symmetric(env):
     lp.apply_to(yd, my)
     a = float(np.dot(x.ravel().astype(np.float64), my.cpu().numpy().ravel()))
     b = float(np.dot(mx.cpu().numpy().ravel().astype(np.float64), y.ravel()))
-    assert abs(a - b) < 1e-5 * np.sqrt(abs(float(np.dot(x.ravel(), mx.cpu().numpy().ravel()))
-                                              * float(np.dot(y.ravel(), my.cpu().numpy().ravel()))))
-    assert float(np.dot(x.ravel().astype(np.float64), mx.cpu().numpy().ravel())) > 0
+    xmx = float(np.dot(x.ravel().astype(np.float64), mx.cpu().numpy().ravel()))
+    ymy = float(np.dot(y.ravel().astype(np.float64), my.cpu().numpy().ravel()))
+    assert abs(a - b) < 1e-5 * np.sqrt(abs(xmx * ymy))
+    assert xmx > 0
+
+
+def test_restrict_prolong(env):
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(13)
+    B, H, W, f = 2, 37, 45, 4
+    x = rng.standard_normal((B, 1, H, W)).astype(np.float32)
+    y = rng.standard_normal((B, 1, -(-H // f), -(-W // f))).astype(np.float32)
+    xd, yd = _t(env, x), _t(env, y)
+    ry = torch.empty_like(yd)
+    mkgpu.check("mk_restrict_box_f32", xd.data_ptr(), B, H, W, f, ry.data_ptr(), device.stream_ptr())
+    assert rel(ry.cpu().numpy(), OL.restrict(x, f)) < 1e-6
+    px = torch.full_like(xd, 2.0)
+    mkgpu.check("mk_prolong_box_f32", yd.data_ptr(), B, H, W, f, px.data_ptr(), 1, device.stream_ptr())
+    assert rel(px.cpu().numpy(), 2.0 + OL.prolong(y, f, H, W)) < 1e-6
 
 
 def test_adam_matches_numpy(env):
@@ -176,13 +198,13 @@ def test_adam_matches_numpy(env):
     assert np.max(np.abs(pt.cpu().numpy() - pr)) < 1e-5
 
 
-def test_energy_forward_backward_vs_oracle(env):
+@pytest.mark.parametrize("arch,spec", [("chain", (1, 2, 1, 3)), ("levels", (1, 2, 4))])
+def test_energy_forward_backward_vs_oracle(env, arch, spec):
     torch, device, mkgpu = env
     from minkasi_amd.learned.train import Trainer
-    dils = (1, 2, 1, 3)
     H, W = 14, 11
     n = H * W
-    p, net, _, _, s, _ = _net_and_data(dils, 1, H, W, seed=5)
+    p, net, _, _, s, _ = _net_and_data(arch, spec, 1, H, W, seed=5)
     rng = np.random.default_rng(8)
     Q = rng.standard_normal((n, n))
     A = Q @ Q.T / n + 0.05 * np.eye(n)
@@ -195,9 +217,9 @@ def test_energy_forward_backward_vs_oracle(env):
     def apply_A(z, out):  # dense test operator
         out.copy_((At @ z.reshape(-1)).reshape(H, W))
 
-    tr = Trainer(p, dils, 1, H, W)
+    tr = Trainer(p, arch, spec, 1, H, W)
     ins = [_t(env, a) for a in (r, e, s2)]
-    print("energy_forward_backward", flush=True)
+    print("energy_forward_backward", arch, spec, flush=True)
     tr.energy_forward_backward(ins[0], ins[1], ins[2], apply_A)
     torch.cuda.synchronize()
     assert abs(tr.loss() - loss) < 1e-4 * loss

@@ -152,3 +152,49 @@ def test_energy_loss_grad_finite_differences():
     lm = net.energy_loss_and_grad(r, e, s, A)[0]
     net.log_alpha = old
     assert abs((lp - lm) / (2 * h) - dla) < 1e-6 * max(1.0, abs(dla))
+
+
+def test_restrict_prolong_adjoint_and_means():
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((2, 1, 13, 10))
+    y = rng.standard_normal((2, 1, 4, 3))
+    rx = L.restrict(x, 4)
+    assert rx.shape == (2, 1, 4, 3)
+    assert abs(rx[1, 0, 1, 2] - x[1, 0, 4:8, 8:10].sum() / 16) < 1e-14  # partial block: zeros count
+    assert abs(np.sum(rx * y) - np.sum(x * L.prolong(y, 4, 13, 10))) < 1e-12
+
+
+def levels_net(rng, factors=(1, 2, 6), ch=3):
+    ws = [rng.standard_normal((ch, 1, 9)) * 0.4 for _ in factors]
+    return L.LevelsNet(ws, factors, log_alpha=-0.2)
+
+
+def test_levels_grad_finite_differences_and_spd():
+    rng = np.random.default_rng(12)
+    net = levels_net(rng)
+    shp = (2, 1, 14, 13)
+    r, e = rng.standard_normal(shp), rng.standard_normal(shp)
+    s = rng.uniform(0.2, 1.5, shp)
+    m = rng.uniform(size=shp)
+    loss, grads, dla = net.loss_and_grad(r, e, s, m)
+    assert abs(loss - np.sum(m * (net.apply(r, s) - e) ** 2)) < 1e-10 * loss
+    h = 1e-6
+    for l in range(len(net.ws)):
+        for idx in [(0, 0, 0), (2, 0, 7)]:
+            old = net.ws[l][idx]
+            net.ws[l][idx] = old + h
+            lp = net.loss_and_grad(r, e, s, m)[0]
+            net.ws[l][idx] = old - h
+            lm = net.loss_and_grad(r, e, s, m)[0]
+            net.ws[l][idx] = old
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - grads[l][idx]) < 1e-6 * max(1.0, abs(fd)), (l, idx, fd, grads[l][idx])
+    H, W = 9, 8
+    s2 = rng.uniform(0.1, 2.0, (H, W))
+    M = np.zeros((H * W, H * W))
+    for k in range(H * W):
+        v = np.zeros(H * W)
+        v[k] = 1.0
+        M[:, k] = net.apply(v.reshape(H, W), s2).ravel()
+    assert np.allclose(M, M.T, atol=1e-12 * np.abs(M).max())
+    assert np.linalg.eigvalsh(M).min() > 0
