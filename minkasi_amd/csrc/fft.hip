@@ -222,6 +222,7 @@ struct Seg {
   int64_t t_off, t_stride;      // T buffer rows (complex)
   int64_t s_off, s_stride;      // spectrum rows (filter: P+1 reals; Bluestein: P complex, stride 0)
   void* s_ptr;                  // device address of this segment's spectrum rows (set at upload)
+  void* tw_ptr;                 // hot K2: fine/coarse twiddle tables of this P (set at create)
   int32_t rows, n, l1, l2, cw, flags;
   int64_t base1, base2, base3;  // first global item of K1, K2, K3
   int32_t per1, per2;           // items per row for K1/K3, K2
@@ -480,6 +481,324 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
   }
 }
 
+// ---- K2, FP32 filter hot path: rows of 2^11 points --------------------------
+// Every workgroup holds exactly two rows: k1 and its partner P1-k1, or, for
+// the first workgroup of a mode, the two self-paired rows k1 = 0 and P1/2.
+// Stage and four-step twiddles come from tables built once per plan in FP64
+// (no transcendental in the loop); the stage shapes are compile-time, so the
+// LDS addresses fold into instruction offsets.  Each thread owns four
+// (k, P-k) pairs of the spectral step (five for thread 0 of workgroup 0).
+constexpr int kHotL2 = 11;
+constexpr int kHotP2 = 1 << kHotL2;
+constexpr int kHotRS = kHotP2 + kHotP2 / 8 + 1;
+constexpr int kHotMaxL1 = 8;                  // P1 <= 256
+constexpr int kFineBits = 10;                 // fine table: 1024 entries of w_2P
+// X rows + fine/coarse tables; the stage twiddles are read from the global
+// table (16 KB, L1-resident), which keeps three workgroups per CU.
+constexpr int kHotLds = (2 * kHotRS + (1 << kFineBits) + (4 << kHotMaxL1)) * 8;
+
+// Compile-time Stockham stage over NROWS padded rows (stride RS) of 2^L2LEN
+// points; NROWS * 2^L2LEN = kMaxWG.
+template <bool INV, int LR, int L2LEN, int LNS, int NROWS, int RS>
+__device__ __forceinline__ void ct_stage(cx<float>* buf, const cx<float>* tw) {
+  constexpr int R = 1 << LR;
+  constexpr int NB = (NROWS << L2LEN) / R / kThreads;
+  static_assert(NB >= 1 && NB * R * kThreads == (NROWS << L2LEN), "stage does not tile the workgroup");
+  constexpr int LQ = L2LEN - LR, Q = 1 << LQ, NS = 1 << LNS;
+  cx<float> v[NB][R];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    const int row = bf >> LQ, j = bf & (Q - 1), k = j & (NS - 1);
+    const cx<float>* base = buf + row * RS;
+#pragma unroll
+    for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * Q)];
+    if (LNS > 0) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        cx<float> w = tw[(r - 1) * NS + k];
+        if (INV) w = cconj(w);
+        v[u][r] = cmul(v[u][r], w);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    const int row = bf >> LQ, j = bf & (Q - 1), k = j & (NS - 1);
+    if (LR == 3) dft8<float, INV>(v[u]);
+    else if (LR == 2) dft4v<float, INV>(v[u]);
+    else dft2<float, INV>(v[u]);
+    cx<float>* base = buf + row * RS;
+    const int idx = ((j >> LNS) << (LNS + LR)) + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) base[pad8(idx + r * NS)] = v[u][r];
+  }
+  __syncthreads();
+}
+
+// FFT of NROWS rows of 2^L2LEN points, stages of radix 8 then 4 or 2; st is
+// the stage-ordered table (fill_stage_table order).
+template <bool INV, int L2LEN, int NROWS, int RS, int LNS = 0, int OFF = 0>
+__device__ __forceinline__ void ct_fft(cx<float>* buf, const cx<float>* st) {
+  if constexpr (LNS < L2LEN) {
+    constexpr int LR = (L2LEN - LNS) >= 3 ? 3 : (L2LEN - LNS);
+    ct_stage<INV, LR, L2LEN, LNS, NROWS, RS>(buf, st + OFF);
+    ct_fft<INV, L2LEN, NROWS, RS, LNS + LR, OFF + ((1 << LR) - 1) * (1 << LNS)>(buf, st);
+  }
+}
+
+__device__ __forceinline__ cx<float> tw2P(const cx<float>* F, const cx<float>* C, int k) {
+  return cmul(F[k & ((1 << kFineBits) - 1)], C[k >> kFineBits]);  // w_2P^k
+}
+
+__global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restrict__ segs, int nseg,
+                                                              cx<float>* __restrict__ tbuf,
+                                                              const cx<float>* __restrict__ stab_g) {
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 2);
+  const Seg sg = segs[s];
+  const int local = (int)((int64_t)blockIdx.x - sg.base2);
+  const int row = local / sg.per2, g = local - row * sg.per2;
+  const int l1 = sg.l1, P1 = 1 << l1;
+  const int P = P1 << kHotL2;
+  const int kA = g == 0 ? 0 : g, kB = g == 0 ? P1 / 2 : P1 - g;
+  cx<float>* X = reinterpret_cast<cx<float>*>(smem_raw);  // [2][kHotRS]
+  cx<float>* F = X + 2 * kHotRS;                           // [1024] w_2P^l
+  cx<float>* Ct = F + (1 << kFineBits);                    // [4 P1] w_2P^(1024 h)
+  const cx<float>* st = stab_g;                            // stage table (global)
+  const cx<float>* twF = reinterpret_cast<const cx<float>*>(sg.tw_ptr);
+  const int nC = 4 * P1;
+  for (int e = threadIdx.x; e < (1 << kFineBits); e += kThreads) F[e] = twF[e];
+  for (int e = threadIdx.x; e < nC; e += kThreads) Ct[e] = twF[(1 << kFineBits) + e];
+  cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  const float* srow = reinterpret_cast<const float*>(sg.s_ptr) + (int64_t)row * sg.s_stride;
+  {
+    cx<float> va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k2 = threadIdx.x + u * kThreads;
+      va[u] = trow[kA * kHotP2 + k2];
+      vb[u] = trow[kB * kHotP2 + k2];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k2 = threadIdx.x + u * kThreads;
+      X[pad8(k2)] = va[u];
+      X[kHotRS + pad8(k2)] = vb[u];
+    }
+  }
+  // this thread's pairs: (slot, k2) <-> (slot_q, k2q)
+  constexpr int NPAIR = 5;
+  int p_sl[NPAIR], p_k2[NPAIR], p_slq[NPAIR], p_k2q[NPAIR];
+  float bb[NPAIR], bq[NPAIR];
+  float bP = 0.f;
+#pragma unroll
+  for (int u = 0; u < NPAIR; u++) {
+    const int pl = threadIdx.x + u * kThreads;
+    int sl = -1, k2 = 0, slq = 0, k2q = 0;
+    if (g != 0) {
+      if (pl < kHotP2) { sl = 0; k2 = pl; slq = 1; k2q = kHotP2 - 1 - pl; }
+    } else if (pl <= kHotP2 / 2) {
+      sl = 0; k2 = pl; slq = 0; k2q = (kHotP2 - pl) & (kHotP2 - 1);
+    } else if (pl < kHotP2 + 1) {
+      sl = 1; k2 = pl - (kHotP2 / 2 + 1); slq = 1; k2q = kHotP2 - 1 - k2;
+    }
+    p_sl[u] = sl; p_k2[u] = k2; p_slq[u] = slq; p_k2q[u] = k2q;
+    bb[u] = bq[u] = 0.f;
+    if (sl >= 0) {
+      bb[u] = srow[(sl ? kB : kA) * kHotP2 + k2];
+      bq[u] = srow[(slq ? kB : kA) * kHotP2 + k2q];
+    }
+  }
+  if (g == 0 && threadIdx.x == 0) bP = srow[P];
+  __syncthreads();
+  ct_fft<false, kHotL2, 2, kHotRS>(X, st);
+#pragma unroll
+  for (int u = 0; u < NPAIR; u++) {
+    const int sl = p_sl[u];
+    if (sl < 0) continue;
+    const int k2 = p_k2[u], slq = p_slq[u], k2q = p_k2q[u];
+    const int kk1 = sl ? kB : kA;
+    const int idx = sl * kHotRS + pad8(k2), pidx = slq * kHotRS + pad8(k2q);
+    const int k = kk1 + P1 * k2;
+    cx<float> C = X[idx], Cq = X[pidx];
+    cx<float> E = cscale(cadd(C, cconj(Cq)), 0.5f);
+    cx<float> Dd = csub(C, cconj(Cq));
+    cx<float> O = {Dd.im * 0.5f, -Dd.re * 0.5f};
+    cx<float> w = tw2P(F, Ct, k);
+    cx<float> A = cadd(E, cmul(w, O));
+    cx<float> Aq = csub(cconj(E), cmul(cconj(w), cconj(O)));
+    cx<float> Y = cscale(A, bb[u]), Yq = cscale(Aq, bq[u]);
+    if (k == 0) {
+      const float yP = (E.re - O.re) * bP;
+      cx<float> Ep = {0.5f * (Y.re + yP), 0.5f * Y.im};
+      cx<float> Op = {0.5f * (Y.re - yP), 0.5f * Y.im};
+      X[idx] = {Ep.re - Op.im, Ep.im + Op.re};
+      continue;
+    }
+    cx<float> Ep = cscale(cadd(Y, cconj(Yq)), 0.5f);
+    cx<float> Op = cmul(cscale(csub(Y, cconj(Yq)), 0.5f), cconj(w));
+    cx<float> Eq = cscale(cadd(Yq, cconj(Y)), 0.5f);
+    cx<float> Oq = cmul(cscale(csub(Yq, cconj(Y)), 0.5f), cx<float>{-w.re, -w.im});
+    X[idx] = {Ep.re - Op.im, Ep.im + Op.re};
+    if (pidx != idx) X[pidx] = {Eq.re - Oq.im, Eq.im + Oq.re};
+  }
+  __syncthreads();
+  ct_fft<true, kHotL2, 2, kHotRS>(X, st);
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int j2 = threadIdx.x + u * kThreads;
+    const int aA = (int)(((int64_t)j2 * kA) & (P - 1)), aB = (int)(((int64_t)j2 * kB) & (P - 1));
+    trow[kA * kHotP2 + j2] = cmul(X[pad8(j2)], cconj(tw2P(F, Ct, 2 * aA)));
+    trow[kB * kHotP2 + j2] = cmul(X[kHotRS + pad8(j2)], cconj(tw2P(F, Ct, 2 * aB)));
+  }
+}
+
+// ---- K1 / K3, FP32 filter hot path: columns of P1 = 2^L1 (rows of 2^11) ----
+// One launch per column length; CW = 4096 / P1 adjacent columns per
+// workgroup, so global reads and writes are runs of CW points.
+template <int L1>
+__global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restrict__ segs, int nseg,
+                                                              const float* __restrict__ in,
+                                                              cx<float>* __restrict__ tbuf,
+                                                              const cx<float>* __restrict__ stab_g) {
+  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = P1 + P1 / 8 + 1;
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 1);
+  const Seg sg = segs[s];
+  const int local = (int)((int64_t)blockIdx.x - sg.base1);
+  const int row = local / sg.per1, jb = local - row * sg.per1;
+  const int P = P1 << kHotL2;
+  const int m = sg.n - 1;
+  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
+  cx<float>* st = buf + CW * RS;                             // [P1] stage table
+  cx<float>* F = st + P1;                                    // [1024] w_2P^l
+  cx<float>* Ct = F + (1 << kFineBits);                      // [4 P1] w_2P^(1024 h)
+  const cx<float>* twF = reinterpret_cast<const cx<float>*>(sg.tw_ptr);
+  for (int e = threadIdx.x; e < P1 - 1; e += kThreads) st[e] = stab_g[e];
+  for (int e = threadIdx.x; e < (1 << kFineBits); e += kThreads) F[e] = twF[e];
+  for (int e = threadIdx.x; e < 4 * P1; e += kThreads) Ct[e] = twF[(1 << kFineBits) + e];
+  const float* x = in + sg.in_off + (int64_t)row * sg.in_stride;
+  const int j2_0 = jb * CW;
+  // real input a_i, i = 2 (P2 j1 + j2) + w, from the even extension on [-m, 2m]
+  constexpr int NV = 2 * kMaxWG / kThreads;  // 16 reals per thread
+  float val[NV];
+#pragma unroll
+  for (int u = 0; u < NV; u++) {
+    const int idx = threadIdx.x + u * kThreads;
+    const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
+    const int i = 2 * (kHotP2 * j1 + j2_0) + w;
+    val[u] = 0.f;
+    if (i <= 3 * m) {
+      const int t = i - m;
+      val[u] = x[t < 0 ? -t : (t <= m ? t : 2 * m - t)];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NV; u++) {
+    const int idx = threadIdx.x + u * kThreads;
+    const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
+    reinterpret_cast<float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] = val[u];
+  }
+  __syncthreads();
+  ct_fft<false, L1, CW, RS>(buf, st);
+  cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+#pragma unroll
+  for (int u = 0; u < kMaxWG / kThreads; u++) {
+    const int idx = threadIdx.x + u * kThreads;
+    const int k1 = idx >> LCW, c = idx & (CW - 1);
+    const int a = (int)(((int64_t)(j2_0 + c) * k1) & (P - 1));
+    trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[c * RS + pad8(k1)], tw2P(F, Ct, 2 * a));
+  }
+}
+
+template <int L1>
+__global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restrict__ segs, int nseg,
+                                                              const cx<float>* __restrict__ tbuf,
+                                                              float* __restrict__ out,
+                                                              const cx<float>* __restrict__ stab_g) {
+  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = P1 + P1 / 8 + 1;
+  extern __shared__ unsigned char smem_raw[];
+  const int s = find_seg(segs, nseg, blockIdx.x, 3);
+  const Seg sg = segs[s];
+  const int local = (int)((int64_t)blockIdx.x - sg.base3);
+  const int row = local / sg.per1, jb = local - row * sg.per1;
+  const int P = P1 << kHotL2;
+  const int m = sg.n - 1;
+  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
+  cx<float>* st = buf + CW * RS;                             // [P1] stage table
+  for (int e = threadIdx.x; e < P1 - 1; e += kThreads) st[e] = stab_g[e];
+  const cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  const int j2_0 = jb * CW;
+  {
+    constexpr int NV = kMaxWG / kThreads;  // 8 points per thread
+    cx<float> v[NV];
+#pragma unroll
+    for (int u = 0; u < NV; u++) {
+      const int idx = threadIdx.x + u * kThreads;
+      v[u] = trow[(idx >> LCW) * kHotP2 + j2_0 + (idx & (CW - 1))];
+    }
+#pragma unroll
+    for (int u = 0; u < NV; u++) {
+      const int idx = threadIdx.x + u * kThreads;
+      buf[(idx & (CW - 1)) * RS + pad8(idx >> LCW)] = v[u];
+    }
+  }
+  __syncthreads();
+  ct_fft<true, L1, CW, RS>(buf, st);
+  const float scale = 1.f / (float)P;
+  const bool halve = sg.flags & 1;
+  float* y = out + sg.out_off + (int64_t)row * sg.out_stride;
+#pragma unroll
+  for (int u = 0; u < 2 * kMaxWG / kThreads; u++) {
+    const int idx = threadIdx.x + u * kThreads;
+    const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
+    const int k = 2 * (kHotP2 * j1 + j2_0) + w - m;
+    if (k >= 0 && k <= m) {
+      float v = reinterpret_cast<const float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] * scale;
+      if (halve && (k == 0 || k == m)) v *= 0.5f;
+      y[k] = v;
+    }
+  }
+}
+
+// LDS of the hot K1 (tile, stage table, fine/coarse tables) and K3 (tile,
+// stage table).
+size_t hot13_lds(int l1, int which) {
+  const int P1 = 1 << l1, CW = kMaxWG / P1, RS = row_stride(P1);
+  return (size_t)(CW * RS + P1 + (which == 1 ? (1 << kFineBits) + 4 * P1 : 0)) * sizeof(cx<float>);
+}
+
+// Twiddle tables of the hot K2 path, computed in FP64: the stage table of a
+// 2^11-point FFT (fill_stage_table order) and, per P, w_2P^l (l < 1024) then
+// w_2P^(1024 h) (h < 2P / 1024).
+__global__ void hot_tables_kernel(cx<float>* out, int64_t P, int stage_l2) {
+  if (stage_l2 > 0) {  // stage table of a 2^stage_l2-point FFT
+    int Ns = 1, rem = stage_l2, off = 0;
+    while (rem > 0) {
+      const int lr = stage_lr(rem), R = 1 << lr, cnt = (R - 1) * Ns, mul = 1 << (rem - lr);
+      for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+        int r = e / Ns + 1, k = e - (r - 1) * Ns;
+        double sn, cs;
+        sincospi(-2.0 * (double)((int64_t)k * r * mul) / (double)(1 << stage_l2), &sn, &cs);
+        out[off + e] = {(float)cs, (float)sn};
+      }
+      off += cnt;
+      Ns <<= lr; rem -= lr;
+    }
+    return;
+  }
+  const int64_t nF = 1 << kFineBits, nC = 2 * P / nF;
+  for (int64_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nF + nC; e += gridDim.x * blockDim.x) {
+    int64_t t = e < nF ? e : (e - nF) * nF;  // exponent of w_2P
+    double sn, cs;
+    sincospi(-(double)t / (double)P, &sn, &cs);
+    out[e] = {(float)cs, (float)sn};
+  }
+}
+
 // ---- K3: inverse column FFTs + output -------------------------------------
 template <typename T, typename TOut, int MODE>
 __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k3_kernel(const Seg* __restrict__ segs, int nseg,
@@ -621,6 +940,13 @@ void allow_lds_all() {
   allow_big_lds(k3_kernel<double, double, kFilter>);
   allow_big_lds(k3_kernel<float, float, kBlue>);
   allow_big_lds(k3_kernel<double, double, kBlue>);
+  allow_big_lds(k2_hot_kernel);
+  allow_big_lds(k1_hot_kernel<1>); allow_big_lds(k1_hot_kernel<2>); allow_big_lds(k1_hot_kernel<3>);
+  allow_big_lds(k1_hot_kernel<4>); allow_big_lds(k1_hot_kernel<5>); allow_big_lds(k1_hot_kernel<6>);
+  allow_big_lds(k1_hot_kernel<7>); allow_big_lds(k1_hot_kernel<8>);
+  allow_big_lds(k3_hot_kernel<1>); allow_big_lds(k3_hot_kernel<2>); allow_big_lds(k3_hot_kernel<3>);
+  allow_big_lds(k3_hot_kernel<4>); allow_big_lds(k3_hot_kernel<5>); allow_big_lds(k3_hot_kernel<6>);
+  allow_big_lds(k3_hot_kernel<7>); allow_big_lds(k3_hot_kernel<8>);
   (void)hipGetLastError();  // an unsupported attribute must not poison later launch checks
 }
 
@@ -662,6 +988,16 @@ struct mk_filt_plan {
   size_t lds1 = 0, lds2 = 0;
   bool have_spec = false;
   std::vector<char> borrowed;
+  // FP32 K2 split: segments on the hot path (rows of 2^11, P1 <= 256) and the
+  // rest, each list with its own K2 numbering; twiddle tables for the hot path
+  Seg* d_hot = nullptr;
+  Seg* d_cold = nullptr;
+  Seg* d_hot13 = nullptr;   // hot segments grouped by column length, K1/K3 numbering per group
+  int nhot = 0, ncold = 0;
+  int64_t n2hot = 0, n2cold = 0, n1cold = 0;
+  int g_off[kHotMaxL1 + 1] = {0}, g_cnt[kHotMaxL1 + 1] = {0};
+  int64_t g_n1[kHotMaxL1 + 1] = {0};
+  void* tw_tab = nullptr;   // stage tables of 2^1..2^11 points (2^l - 2 offsets), then per-P fine/coarse tables
 };
 
 namespace {
@@ -678,17 +1014,107 @@ int ensure_tbuf(const mk_filt_plan* cp) {
   return MK_OK;
 }
 
+bool is_hot(const mk_filt_plan* p, const Seg& s) {
+  return p->elem_bytes == 4 && s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1 && s.tw_ptr != nullptr;
+}
+
 int upload(mk_filt_plan* p) {
   if (p->nseg && hipMemcpy(p->d_segs, p->h_segs.data(), p->nseg * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) {
+    mk::set_error("filter plan: descriptor upload failed");
+    return MK_ERR_HIP;
+  }
+  if (p->elem_bytes != 4) return MK_OK;
+  std::vector<Seg> hot, cold, hot13;
+  int64_t b_hot = 0, b_cold2 = 0, b_cold1 = 0;
+  for (const Seg& s0 : p->h_segs) {
+    Seg s = s0;
+    if (is_hot(p, s)) {
+      s.per2 = 1 << (s.l1 - 1);
+      s.base2 = b_hot;
+      b_hot += (int64_t)s.rows * s.per2;
+      hot.push_back(s);
+    } else {
+      s.base2 = b_cold2;
+      b_cold2 += (int64_t)s.rows * s.per2;
+      s.base1 = s.base3 = b_cold1;
+      b_cold1 += (int64_t)s.rows * s.per1;
+      cold.push_back(s);
+    }
+  }
+  for (int l = 0; l <= kHotMaxL1; l++) {
+    p->g_off[l] = (int)hot13.size();
+    int64_t b = 0;
+    for (const Seg& s0 : hot)
+      if (s0.l1 == l) {
+        Seg s = s0;
+        s.base1 = s.base3 = b;
+        b += (int64_t)s.rows * s.per1;
+        hot13.push_back(s);
+      }
+    p->g_cnt[l] = (int)hot13.size() - p->g_off[l];
+    p->g_n1[l] = b;
+  }
+  p->nhot = (int)hot.size();
+  p->ncold = (int)cold.size();
+  p->n2hot = b_hot;
+  p->n2cold = b_cold2;
+  p->n1cold = b_cold1;
+  if ((p->nhot && hipMemcpy(p->d_hot, hot.data(), hot.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) ||
+      (p->nhot &&
+       hipMemcpy(p->d_hot13, hot13.data(), hot13.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess) ||
+      (p->ncold &&
+       hipMemcpy(p->d_cold, cold.data(), cold.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess)) {
     mk::set_error("filter plan: descriptor upload failed");
     return MK_ERR_HIP;
   }
   return MK_OK;
 }
 
+const cx<float>* stage_tab(const mk_filt_plan* p, int l) { return (const cx<float>*)p->tw_tab + ((1 << l) - 2); }
+
+// Builds the hot-path twiddle tables of an FP32 plan and points each hot
+// segment at its P's tables.
+int build_hot_tables(mk_filt_plan* p) {
+  std::vector<int64_t> Ps;
+  for (const Seg& s : p->h_segs)
+    if (s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1) {
+      int64_t P = int64_t(1) << (s.l1 + s.l2);
+      if (std::find(Ps.begin(), Ps.end(), P) == Ps.end()) Ps.push_back(P);
+    }
+  if (Ps.empty()) return MK_OK;
+  std::vector<int64_t> offs;
+  int64_t tot = 2 * kHotP2;  // stage tables of 2^1 .. 2^11 points first
+  for (int64_t P : Ps) {
+    offs.push_back(tot);
+    tot += (1 << kFineBits) + 2 * P / (1 << kFineBits);
+  }
+  if (hipMalloc(&p->tw_tab, tot * sizeof(cx<float>)) != hipSuccess) {
+    p->tw_tab = nullptr;
+    mk::set_error("filter plan: twiddle table allocation failed");
+    return MK_ERR_ALLOC;
+  }
+  cx<float>* base = (cx<float>*)p->tw_tab;
+  for (int l = 1; l <= kHotL2; l++) hot_tables_kernel<<<8, 256>>>(base + ((1 << l) - 2), 0, l);
+  for (size_t i = 0; i < Ps.size(); i++) hot_tables_kernel<<<64, 256>>>(base + offs[i], Ps[i], 0);
+  int r = mk::check_launch("hot_tables_kernel");
+  if (r) return r;
+  MK_HIP(hipDeviceSynchronize());
+  for (Seg& s : p->h_segs)
+    if (s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1) {
+      int64_t P = int64_t(1) << (s.l1 + s.l2);
+      size_t i = std::find(Ps.begin(), Ps.end(), P) - Ps.begin();
+      s.tw_ptr = base + offs[i];
+    }
+  return MK_OK;
+}
+
 void free_filt(mk_filt_plan* p) {
   if (!p) return;
   if (p->d_segs) (void)hipFree(p->d_segs);
+  if (p->d_hot) (void)hipFree(p->d_hot);
+  if (p->d_hot13) (void)hipFree(p->d_hot13);
+  if (p->d_cold) (void)hipFree(p->d_cold);
+  if (p->tw_tab) (void)hipFree(p->tw_tab);
   if (p->tbuf) (void)hipFree(p->tbuf);
   if (p->spec) (void)hipFree(p->spec);
   delete p;
@@ -742,8 +1168,55 @@ void launch_fspec(mk_filt_plan* p, const void* wdct, void* tb, size_t lds1, size
   k2_kernel<T, TS, kFSpec><<<(unsigned)p->n2, kThreads, lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)tb);
 }
 
+template <int L1>
+void launch_hot13(const mk_filt_plan* p, int which, const float* in, float* out, hipStream_t st) {
+  const Seg* segs = p->d_hot13 + p->g_off[L1];
+  const unsigned grid = (unsigned)p->g_n1[L1];
+  if (which == 1)
+    k1_hot_kernel<L1><<<grid, kThreads, hot13_lds(L1, 1), st>>>(segs, p->g_cnt[L1], in, (cx<float>*)p->tbuf,
+                                                             stage_tab(p, L1));
+  else
+    k3_hot_kernel<L1><<<grid, kThreads, hot13_lds(L1, 3), st>>>(segs, p->g_cnt[L1], (const cx<float>*)p->tbuf, out,
+                                                             stage_tab(p, L1));
+}
+
+void launch_hot13_all(const mk_filt_plan* p, int which, const float* in, float* out, hipStream_t st) {
+  for (int l = 1; l <= kHotMaxL1; l++) {
+    if (!p->g_cnt[l] || !p->g_n1[l]) continue;
+    switch (l) {
+      case 1: launch_hot13<1>(p, which, in, out, st); break;
+      case 2: launch_hot13<2>(p, which, in, out, st); break;
+      case 3: launch_hot13<3>(p, which, in, out, st); break;
+      case 4: launch_hot13<4>(p, which, in, out, st); break;
+      case 5: launch_hot13<5>(p, which, in, out, st); break;
+      case 6: launch_hot13<6>(p, which, in, out, st); break;
+      case 7: launch_hot13<7>(p, which, in, out, st); break;
+      default: launch_hot13<8>(p, which, in, out, st); break;
+    }
+  }
+}
+
 template <typename T, typename TIO>
 void launch_filter(const mk_filt_plan* p, const TIO* in, TIO* out, hipStream_t st) {
+  if constexpr (sizeof(T) == 4) {
+    if (p->nhot + p->ncold == p->nseg) {
+      launch_hot13_all(p, 1, in, out, st);
+      if (p->n1cold)
+        k1_kernel<T, TIO, kFilter><<<(unsigned)p->n1cold, kThreads, p->lds1, st>>>(p->d_cold, p->ncold, in,
+                                                                                  (cx<T>*)p->tbuf);
+      if (p->n2hot)
+        k2_hot_kernel<<<(unsigned)p->n2hot, kThreads, kHotLds, st>>>(p->d_hot, p->nhot, (cx<float>*)p->tbuf,
+                                                                    stage_tab(p, kHotL2));
+      if (p->n2cold)
+        k2_kernel<T, T, kFilter><<<(unsigned)p->n2cold, kThreads, p->lds2, st>>>(p->d_cold, p->ncold,
+                                                                                (cx<T>*)p->tbuf);
+      launch_hot13_all(p, 3, in, out, st);
+      if (p->n1cold)
+        k3_kernel<T, TIO, kFilter><<<(unsigned)p->n1cold, kThreads, p->lds1, st>>>(p->d_cold, p->ncold,
+                                                                                  (const cx<T>*)p->tbuf, out);
+      return;
+    }
+  }
   k1_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg, in, (cx<T>*)p->tbuf);
   k2_kernel<T, T, kFilter><<<(unsigned)p->n2, kThreads, p->lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)p->tbuf);
   k3_kernel<T, TIO, kFilter><<<(unsigned)p->n1, kThreads, p->lds1, st>>>(p->d_segs, p->nseg,
@@ -767,12 +1240,19 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
   if (st) { free_filt(p); return st; }
   MK_REQUIRE(p->n1 < INT32_MAX && p->n2 < INT32_MAX, (free_filt(p), MK_ERR_ARG), "too many work items");
   if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
+      hipMalloc((void**)&p->d_hot, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
+      hipMalloc((void**)&p->d_cold, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
+      hipMalloc((void**)&p->d_hot13, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
       hipMalloc(&p->spec, std::max<int64_t>(1, p->s_elems) * elem_bytes) != hipSuccess) {
     free_filt(p);
     mk::set_error("filter plan: device allocation failed");
     return MK_ERR_ALLOC;
   }
   for (auto& sg : p->h_segs) sg.s_ptr = (unsigned char*)p->spec + sg.s_off * elem_bytes;
+  if (elem_bytes == 4 && getenv("MK_FFT_NO_HOT") == nullptr) {
+    st = build_hot_tables(p);
+    if (st) { free_filt(p); return st; }
+  }
   if (upload(p)) {
     free_filt(p);
     return MK_ERR_HIP;

@@ -148,3 +148,21 @@ def test_filter_is_symmetric_and_kills_constants(env):
         assert np.abs(yt.cpu().numpy()).max() < 1e-9 * 3.0 * n
     finally:
         mkgpu.check("mk_filt_plan_destroy", h)
+
+
+def test_filter_fp32_hot_and_general_paths_agree(env):
+    """Rows of every FP32 column count P1 = 2..256 (n = 2500 .. 200001) take
+    the specialised 2^11-point K2; MK_FFT_NO_HOT forces the general K2.
+    Both match the FP64 reference to the FP32 tolerance and each other."""
+    import os
+    segs = [(2, 2500), (1, 5001), (3, 20001), (1, 99999), (2, 200001), (2, 1001)]
+    out_hot, ref, _ = _filter(env, segs, np.float32, halve=True, seed=7)
+    os.environ["MK_FFT_NO_HOT"] = "1"
+    try:
+        out_gen, ref2, _ = _filter(env, segs, np.float32, halve=True, seed=7)
+    finally:
+        del os.environ["MK_FFT_NO_HOT"]
+    assert np.array_equal(ref, ref2)
+    assert rel_err(out_hot, ref) < 5e-6
+    assert rel_err(out_gen, ref) < 5e-6
+    assert rel_err(out_hot, out_gen) < 5e-6
