@@ -76,12 +76,16 @@ constexpr int kThreads = 512;                  // threads per FFT workgroup
 constexpr int kMaxWG = 4096;                   // complex points per workgroup in an LDS FFT (8 per thread)
 constexpr int kMaxLds = 144 * 1024;            // dynamic LDS budget per workgroup (gfx950 has 160 KB)
 
-// LDS rows are padded: element e of a row lives at e + e/8 and rows are
+// LDS rows are padded: element e of a row lives at e + e/16 (measured best of
+// e/8, e/16, e/32 and none on gfx950) and rows are
 // row_stride(len) apart (odd), so the strided Stockham writes of the early
 // radix-8 stages, the row-parallel writes of K1/K3 and the unit-stride reads
 // all map to distinct banks.
-__host__ __device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
-__host__ __device__ __forceinline__ int row_stride(int len) { return len + (len >> 3) + 1; }
+#ifndef MK_FFT_PAD_SHIFT
+#define MK_FFT_PAD_SHIFT 4
+#endif
+constexpr __host__ __device__ __forceinline__ int pad8(int e) { return e + (e >> MK_FFT_PAD_SHIFT); }
+constexpr __host__ __device__ __forceinline__ int row_stride(int len) { return len + (len >> MK_FFT_PAD_SHIFT) + 1; }
 
 // In-register DFTs (natural order in and out).
 template <typename T, bool INV>
@@ -490,7 +494,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
 // (k, P-k) pairs of the spectral step (five for thread 0 of workgroup 0).
 constexpr int kHotL2 = 11;
 constexpr int kHotP2 = 1 << kHotL2;
-constexpr int kHotRS = kHotP2 + kHotP2 / 8 + 1;
+constexpr int kHotRS = row_stride(kHotP2);
 constexpr int kHotMaxL1 = 8;                  // P1 <= 256
 constexpr int kFineBits = 10;                 // fine table: 1024 entries of w_2P
 // X rows + fine/coarse tables; the stage twiddles are read from the global
@@ -664,7 +668,7 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
                                                               const float* __restrict__ in,
                                                               cx<float>* __restrict__ tbuf,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = P1 + P1 / 8 + 1;
+  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = row_stride(P1);
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 1);
   const Seg sg = segs[s];
@@ -719,7 +723,7 @@ __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restri
                                                               const cx<float>* __restrict__ tbuf,
                                                               float* __restrict__ out,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = P1 + P1 / 8 + 1;
+  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = row_stride(P1);
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 3);
   const Seg sg = segs[s];
