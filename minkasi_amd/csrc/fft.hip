@@ -618,6 +618,11 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
   }
   if (g == 0 && threadIdx.x == 0) bP = srow[P];
   __syncthreads();
+  // twiddles factor along k2 = tid + 512 u:  w_2P^(k1 + P1 k2) =
+  // w_2P^k1 * w_4096^tid * w_8^u, and w_P^(k1 j2) = w_P^(k1 tid) * (w_P^(512 k1))^u
+  cx<float> w4096t;
+  sincospif(-(float)threadIdx.x * (1.f / 2048.f), &w4096t.im, &w4096t.re);
+  const cx<float> wkA = tw2P(F, Ct, kA), wkB = tw2P(F, Ct, kB);
   ct_fft<false, kHotL2, 2, kHotRS>(X, st);
 #pragma unroll
   for (int u = 0; u < NPAIR; u++) {
@@ -631,7 +636,14 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
     cx<float> E = cscale(cadd(C, cconj(Cq)), 0.5f);
     cx<float> Dd = csub(C, cconj(Cq));
     cx<float> O = {Dd.im * 0.5f, -Dd.re * 0.5f};
-    cx<float> w = tw2P(F, Ct, k);
+    cx<float> w;
+    if (k2 == (int)threadIdx.x + u * kThreads) {  // every pair but those of the row k1 = P1/2 of workgroup 0
+      constexpr float c8 = 0.70710678118654752440f;
+      const cx<float> w8[4] = {{1.f, 0.f}, {c8, -c8}, {0.f, -1.f}, {-c8, -c8}};
+      w = cmul(cmul(sl ? wkB : wkA, w4096t), w8[u & 3]);
+    } else {
+      w = tw2P(F, Ct, k);
+    }
     cx<float> A = cadd(E, cmul(w, O));
     cx<float> Aq = csub(cconj(E), cmul(cconj(w), cconj(O)));
     cx<float> Y = cscale(A, bb[u]), Yq = cscale(Aq, bq[u]);
@@ -651,12 +663,17 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
   }
   __syncthreads();
   ct_fft<true, kHotL2, 2, kHotRS>(X, st);
+  cx<float> tA = tw2P(F, Ct, 2 * (int)(((int64_t)kA * threadIdx.x) & (P - 1)));
+  cx<float> tB = tw2P(F, Ct, 2 * (int)(((int64_t)kB * threadIdx.x) & (P - 1)));
+  const cx<float> sA = tw2P(F, Ct, 2 * (int)(((int64_t)kA * kThreads) & (P - 1)));
+  const cx<float> sB = tw2P(F, Ct, 2 * (int)(((int64_t)kB * kThreads) & (P - 1)));
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j2 = threadIdx.x + u * kThreads;
-    const int aA = (int)(((int64_t)j2 * kA) & (P - 1)), aB = (int)(((int64_t)j2 * kB) & (P - 1));
-    trow[kA * kHotP2 + j2] = cmul(X[pad8(j2)], cconj(tw2P(F, Ct, 2 * aA)));
-    trow[kB * kHotP2 + j2] = cmul(X[kHotRS + pad8(j2)], cconj(tw2P(F, Ct, 2 * aB)));
+    trow[kA * kHotP2 + j2] = cmul(X[pad8(j2)], cconj(tA));
+    trow[kB * kHotP2 + j2] = cmul(X[kHotRS + pad8(j2)], cconj(tB));
+    tA = cmul(tA, sA);
+    tB = cmul(tB, sB);
   }
 }
 

@@ -44,7 +44,8 @@ def test_rotation_batched_segments(env):
     torch, device, mkgpu = env
     from minkasi_amd.lib.plans import RotPlan, SplitMatrix
     rng = np.random.default_rng(1)
-    shapes = [(64, 1000), (200, 3001), (130, 129)]
+    # one plan mixing every full-height class width and the general kernel (ndet > 512)
+    shapes = [(64, 1000), (200, 3001), (130, 129), (600, 257), (33, 77), (448, 500)]
     vs, ds, segs, off = [], [], [], 0
     for ndet, n in shapes:
         v = _orth(rng, ndet)
