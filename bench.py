@@ -116,21 +116,14 @@ TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_traffic_pcg_64tod_2048.txt")
 def measured_traffic(stage):
     """HBM bytes per launch of a stage from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated), or None."""
-    kernels = {"filter": ("k1_kernel<float, float, 0>", "k2_kernel<float, float, 0>", "k3_kernel<float, float, 0>"),
-               "map2tod": ("map2tod_kernel<float>",), "tod2map": ("t2m_items_kernel<float, float>",
-                                                               "t2m_tiles_kernel<float, float>"),
-               "rot_v": ("rot_bf16_kernel<6>",), "rot_vT": ("rot_bf16_kernel<6>",)}.get(stage)
-    if not kernels or not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(TRAFFIC_FILE):
         return None
-    found = {}
     with open(TRAFFIC_FILE) as f:
         for line in f:
-            if line.startswith("bytes_per_dispatch"):
-                _, name, val = line.rstrip("\n").split("\t")
-                found[name] = float(val)
-    if not all(k in found for k in kernels):
-        return None
-    return int(sum(found[k] for k in kernels))
+            parts = line.rstrip("\n").split("\t")
+            if len(parts) == 3 and parts[0] == "stage" and parts[1] == stage:
+                return int(float(parts[2]))
+    return None
 
 
 def copy_probe(nbytes):

@@ -341,7 +341,12 @@ def main(argv=None):
                 raise ValueError("--init weights are %s%s, not %s%s" % (a0, s0, arch, spec))
         else:
             init = np.load(args.init, allow_pickle=False)
-    if args.loss == "energy":
+    if args.seeds == 0:
+        if init is None:
+            raise ValueError("--seeds 0 (no training) needs --init weights")
+        params, hist, meta = np.array(init, dtype=np.float32), [], dict(loss="none", arch=arch, spec=list(spec),
+                                                                         init=args.init)
+    elif args.loss == "energy":
         params, hist, meta = train_energy(args.config, args.seeds, args.ntod, args.steps_per_seed, arch, spec,
                                           args.lr, args.seed, args.init_frac, args.max_iter, args.tol_final,
                                           init=init)
