@@ -295,6 +295,13 @@ int mk_restrict_box_f32(const float *x, int32_t B, int32_t H, int32_t W,
                         int32_t f, float *y, void *stream);
 int mk_prolong_box_f32(const float *y, int32_t B, int32_t H, int32_t W,
                        int32_t f, float *x, int32_t accumulate, void *stream);
+/* Full-weighting (tent) restriction by f and its adjoint (bilinear
+ * prolongation): coarse point I is centred on fine position f I + (f-1)/2,
+ * fine pixel i weighs max(0, 2f - |2i - (2fI + f - 1)|) / (2 f^2) per axis. */
+int mk_restrict_tent_f32(const float *x, int32_t B, int32_t H, int32_t W,
+                         int32_t f, float *y, void *stream);
+int mk_prolong_tent_f32(const float *y, int32_t B, int32_t H, int32_t W,
+                        int32_t f, float *x, int32_t accumulate, void *stream);
 /* z = s * (exp(log_alpha[0]) * u + g); g may be NULL (Jacobi-like). */
 int mk_precon_finish_f32(float *z, const float *s, const float *log_alpha,
                          const float *u, const float *g, int64_t n,

@@ -320,6 +320,62 @@ __global__ void __launch_bounds__(256) prolong_kernel(const float* __restrict__ 
   }
 }
 
+// Full-weighting (tent) restriction by f: coarse point I is centred on fine
+// position f I + (f-1)/2 and fine pixel i weighs max(0, 2f - |d|) / (2 f^2),
+// d = 2i - (2 f I + f - 1), per axis (the weights of a coarse point sum to
+// 1 in the interior).  Its adjoint is bilinear interpolation, so coarse
+// corrections come back smooth instead of blocky.
+__device__ __forceinline__ float tent_w(int i, int I, int f) {
+  int d = 2 * i - (2 * f * I + f - 1);
+  d = d < 0 ? -d : d;
+  return d < 2 * f ? (float)(2 * f - d) : 0.f;
+}
+
+__global__ void __launch_bounds__(256) restrict_tent_kernel(const float* __restrict__ x, int H, int W, int f, int Hc,
+                                                            int Wc, int64_t total, float* __restrict__ y) {
+  const float norm = 1.f / (4.f * (float)f * (float)f * (float)f * (float)f);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = k / ((int64_t)Hc * Wc);
+    int rem = (int)(k - b * Hc * Wc);
+    int I = rem / Wc, J = rem - I * Wc;
+    const float* xb = x + b * (int64_t)H * W;
+    const int i0 = max(0, f * I - f / 2 - 1), i1 = min(H - 1, f * I + f + f / 2);
+    const int j0 = max(0, f * J - f / 2 - 1), j1 = min(W - 1, f * J + f + f / 2);
+    float s = 0.f;
+    for (int i = i0; i <= i1; i++) {
+      const float wi = tent_w(i, I, f);
+      if (wi == 0.f) continue;
+      float r = 0.f;
+      for (int j = j0; j <= j1; j++) r = fmaf(tent_w(j, J, f), xb[(int64_t)i * W + j], r);
+      s = fmaf(wi, r, s);
+    }
+    y[k] = s * norm;
+  }
+}
+
+__global__ void __launch_bounds__(256) prolong_tent_kernel(const float* __restrict__ y, int Hc, int Wc, int f, int H,
+                                                           int W, int64_t total, float* __restrict__ x,
+                                                           int accumulate) {
+  const float norm = 1.f / (4.f * (float)f * (float)f * (float)f * (float)f);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = k / ((int64_t)H * W);
+    int rem = (int)(k - b * H * W);
+    int i = rem / W, j = rem - i * W;
+    const float* yb = y + b * (int64_t)Hc * Wc;
+    // coarse points whose support holds i: I with |2i - (2fI + f - 1)| < 2f
+    const int Ilo = max(0, (2 * i - 3 * f + 1) / (2 * f) - 1), Ihi = min(Hc - 1, (2 * i + f + 1) / (2 * f) + 1);
+    const int Jlo = max(0, (2 * j - 3 * f + 1) / (2 * f) - 1), Jhi = min(Wc - 1, (2 * j + f + 1) / (2 * f) + 1);
+    float v = 0.f;
+    for (int I = Ilo; I <= Ihi; I++) {
+      const float wi = tent_w(i, I, f);
+      if (wi == 0.f) continue;
+      for (int J = Jlo; J <= Jhi; J++) v = fmaf(wi * tent_w(j, J, f), yb[(int64_t)I * Wc + J], v);
+    }
+    v *= norm;
+    x[k] = accumulate ? x[k] + v : v;
+  }
+}
+
 // Adam on a flat FP32 parameter vector.  Element n-1 is log(alpha); its
 // gradient is alpha * dL/dalpha, taken from dalpha (FP64, on the device).
 __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -487,4 +543,25 @@ extern "C" int mk_prolong_box_f32(const float* y, int32_t B, int32_t H, int32_t 
   prolong_kernel<<<mk::grid_for(total, 256), 256, 0, mk::as_stream(stream)>>>(y, Hc, Wc, f, H, W, total, x,
                                                                                accumulate);
   return mk::check_launch("prolong_kernel");
+}
+
+extern "C" int mk_restrict_tent_f32(const float* x, int32_t B, int32_t H, int32_t W, int32_t f, float* y,
+                                    void* stream) {
+  MK_REQUIRE(x && y && x != y, MK_ERR_ARG, "restrict_tent: bad pointers");
+  MK_REQUIRE(B > 0 && H > 0 && W > 0 && f >= 1 && f <= 64, MK_ERR_ARG, "restrict_tent: bad shape");
+  int Hc = (int)mk::cdiv(H, f), Wc = (int)mk::cdiv(W, f);
+  int64_t total = (int64_t)B * Hc * Wc;
+  restrict_tent_kernel<<<mk::grid_for(total, 256), 256, 0, mk::as_stream(stream)>>>(x, H, W, f, Hc, Wc, total, y);
+  return mk::check_launch("restrict_tent_kernel");
+}
+
+extern "C" int mk_prolong_tent_f32(const float* y, int32_t B, int32_t H, int32_t W, int32_t f, float* x,
+                                   int32_t accumulate, void* stream) {
+  MK_REQUIRE(x && y && x != y, MK_ERR_ARG, "prolong_tent: bad pointers");
+  MK_REQUIRE(B > 0 && H > 0 && W > 0 && f >= 1 && f <= 64, MK_ERR_ARG, "prolong_tent: bad shape");
+  int Hc = (int)mk::cdiv(H, f), Wc = (int)mk::cdiv(W, f);
+  int64_t total = (int64_t)B * H * W;
+  prolong_tent_kernel<<<mk::grid_for(total, 256), 256, 0, mk::as_stream(stream)>>>(y, Hc, Wc, f, H, W, total, x,
+                                                                                    accumulate);
+  return mk::check_launch("prolong_tent_kernel");
 }

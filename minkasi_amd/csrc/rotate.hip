@@ -106,26 +106,39 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
       }
     }
     // B tile: 32 rows (k) x 128 columns (n), FP32 -> bf16 pieces, stored [n][k].
-    // All 16 loads of a thread are issued before any is used.
+    // Each thread takes two (column, 8-k strip) items: the lanes of a load
+    // read consecutive columns of one row (coalesced), and each piece of a
+    // strip goes to LDS as one 16-byte store.  All 16 loads are issued first.
     {
-      constexpr int PER = BK * BN / 256;
-      float xv[PER];
+      constexpr int ITEMS = BN * (BK / 8), PER = ITEMS / 256;
+      float xv[PER][8];
 #pragma unroll
-      for (int u = 0; u < PER; u++) {
-        int idx = threadIdx.x + u * 256;
-        int kk = idx >> 7, nn = idx & (BN - 1);
-        int gk = k0 + kk, gn = n0 + nn;
-        xv[u] = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
+      for (int v = 0; v < PER; v++) {
+        const int it = threadIdx.x + v * 256;
+        const int nn = it & (BN - 1), kg = it >> 7;
+        const int gn = n0 + nn;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          const int gk = k0 + kg * 8 + kk;
+          xv[v][kk] = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < PER; u++) {
-        int idx = threadIdx.x + u * 256;
-        int kk = idx >> 7, nn = idx & (BN - 1);
-        __bf16 h, m, l;
-        split3_bf16(xv[u], &h, &m, &l);
-        Bs[0][nn][kk] = h;
-        Bs[1][nn][kk] = m;
-        if (NPL == 3) Bs[NPL - 1][nn][kk] = l;
+      for (int v = 0; v < PER; v++) {
+        const int it = threadIdx.x + v * 256;
+        const int nn = it & (BN - 1), kg = it >> 7;
+        bf16x8 ph, pm, plo;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          __bf16 h, m, l;
+          split3_bf16(xv[v][kk], &h, &m, &l);
+          ph[kk] = h;
+          pm[kk] = m;
+          plo[kk] = l;
+        }
+        *reinterpret_cast<bf16x8*>(&Bs[0][nn][kg * 8]) = ph;
+        *reinterpret_cast<bf16x8*>(&Bs[1][nn][kg * 8]) = pm;
+        if (NPL == 3) *reinterpret_cast<bf16x8*>(&Bs[NPL - 1][nn][kg * 8]) = plo;
       }
     }
     __syncthreads();

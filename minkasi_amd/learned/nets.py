@@ -15,9 +15,12 @@ levels  K u = [C_0 u; C_1 R_1 u; C_2 R_2 R_1 u; C_3 R_3 R_2 R_1 u]: four
         an additive multilevel (BPX-like) operator: a learned PSD 5x5
         stencil per level, its coarse levels reaching the map-wide modes
         a chain of 3x3 taps cannot.
+tent    as levels with full-weighting restriction (tent weights) in place of
+        the box average, so the prolongation R^T is bilinear interpolation
+        and coarse corrections come back smooth.
 
 Every operation is a libmkgpu call (mk_conv3x3_f32, mk_conv3x3_wgrad_f32,
-mk_conv3x3_adjoint_weights, mk_restrict_box_f32, mk_prolong_box_f32).
+mk_conv3x3_adjoint_weights, mk_restrict_{box,tent}_f32, mk_prolong_{box,tent}_f32).
 """
 import numpy as np
 import torch
@@ -31,7 +34,7 @@ CH = 16
 def layer_shapes(arch, nlayer):
     if arch == "chain":
         return [(CH, 1)] + [(CH, CH)] * (nlayer - 1)
-    if arch == "levels":
+    if arch in ("levels", "tent"):
         return [(CH, 1)] * nlayer
     raise ValueError("unknown architecture %r" % (arch,))
 
@@ -157,8 +160,10 @@ class ChainNet(_Base):
 
 
 class LevelsNet(_Base):
-    def __init__(self, params, factors, B, H, W, train=False):
-        super().__init__("levels", factors, params, B, H, W)
+    def __init__(self, params, factors, B, H, W, train=False, arch="levels"):
+        super().__init__(arch, factors, params, B, H, W)
+        kind = "tent" if arch == "tent" else "box"
+        self._rname, self._pname = "mk_restrict_%s_f32" % kind, "mk_prolong_%s_f32" % kind
         f = self.spec
         if f[0] != 1 or any(f[i] % f[i - 1] for i in range(1, len(f))):
             raise ValueError("level factors must start at 1 and each divide the next: %s" % (f,))
@@ -181,12 +186,11 @@ class LevelsNet(_Base):
 
     def _restrict(self, x, l, y):
         h, w = self.dims[l - 1]
-        mkgpu.check("mk_restrict_box_f32", x.data_ptr(), self.B, h, w, self.steps[l], y.data_ptr(),
-                    dev.stream_ptr())
+        mkgpu.check(self._rname, x.data_ptr(), self.B, h, w, self.steps[l], y.data_ptr(), dev.stream_ptr())
 
     def _prolong(self, y, l, x, accumulate):
         h, w = self.dims[l - 1]
-        mkgpu.check("mk_prolong_box_f32", y.data_ptr(), self.B, h, w, self.steps[l], x.data_ptr(), int(accumulate),
+        mkgpu.check(self._pname, y.data_ptr(), self.B, h, w, self.steps[l], x.data_ptr(), int(accumulate),
                     dev.stream_ptr())
 
     def forward(self, u, g0):
@@ -220,6 +224,6 @@ class LevelsNet(_Base):
 def make_net(arch, spec, params, B, H, W, train=False):
     if arch == "chain":
         return ChainNet(params, spec, B, H, W, train)
-    if arch == "levels":
-        return LevelsNet(params, spec, B, H, W, train)
+    if arch in ("levels", "tent"):
+        return LevelsNet(params, spec, B, H, W, train, arch)
     raise ValueError("unknown architecture %r" % (arch,))

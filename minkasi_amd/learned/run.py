@@ -26,7 +26,7 @@ def log(*a):
 def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", type=int, default=4)
-    ap.add_argument("--arch", default="levels", choices=["levels", "chain"])
+    ap.add_argument("--arch", default="levels", choices=["levels", "tent", "chain"])
     ap.add_argument("--spec", default="1,4,16,64", help="level factors (levels) or dilations (chain)")
     ap.add_argument("--loss", default="energy", choices=["energy", "crop"])
     ap.add_argument("--seeds", type=int, default=32)
@@ -45,6 +45,7 @@ def parse(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (.npz from save or .npy params)")
     ap.add_argument("--calibrate", default="", help="K^T K scale factors tried on a held-out problem, e.g. "
                     "0.25,0.5,1,2 (empty: none)")
+    ap.add_argument("--calib-ntod", type=int, default=64, help="TODs of the held-out calibration problem")
     ap.add_argument("--eval", action="store_true")
     ap.add_argument("--eval-ntod", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -363,7 +364,7 @@ def main(argv=None):
         from ..synth import tod_shapes
         factors = tuple(float(c) for c in args.calibrate.split(","))
         # held out from both the training problems and the benchmark problem
-        Pc = build_problem(args.config, shapes=tod_shapes(args.config, args.ntod, seed=29999),
+        Pc = build_problem(args.config, shapes=tod_shapes(args.config, args.calib_ntod, seed=29999),
                            seed_base=9_000_000)
         params, cbest, rows = calibrate(Pc, params, arch, spec, args.tol, args.maxiter, factors)
         meta.update(calibration=dict(factors=factors, iterations=[r[1] for r in rows], chosen=cbest))

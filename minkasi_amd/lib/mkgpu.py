@@ -42,6 +42,8 @@ _SIGS = {
     "mk_precon_finish_f32": ([P, P, P, P, P, I64, P], I),
     "mk_restrict_box_f32": ([P, I32, I32, I32, I32, P, P], I),
     "mk_prolong_box_f32": ([P, I32, I32, I32, I32, P, I32, P], I),
+    "mk_restrict_tent_f32": ([P, I32, I32, I32, I32, P, P], I),
+    "mk_prolong_tent_f32": ([P, I32, I32, I32, I32, P, I32, P], I),
     "mk_precon_loss_workspace_bytes": ([], I64),
     "mk_precon_aloss_f32": ([P, P, P, P, P, P, I64, P, P, P, I64, P], I),
     "mk_precon_loss_f32": ([P, P, P, P, P, P, I64, P, P, P, I64, P], I),

@@ -217,3 +217,42 @@ class LevelsNet(Net):
         for ul, ql, w in zip(us, qs, self.ws):
             grads.append(wgrad(conv3x3(ul, w, 1), ql, 1) + wgrad(conv3x3(ql, w, 1), ul, 1))
         return grads, dalpha * alpha
+
+
+def tent_matrix(H, f):
+    """(ceil(H/f), H) full-weighting weights: w[I, i] = max(0, 2f - |2i - (2fI + f - 1)|) / (2 f^2)."""
+    Hc = -(-H // f)
+    I = np.arange(Hc)[:, None]
+    i = np.arange(H)[None, :]
+    d = np.abs(2 * i - (2 * f * I + f - 1))
+    return np.where(d < 2 * f, (2 * f - d) / (2.0 * f * f), 0.0)
+
+
+def restrict_tent(x, f):
+    x = np.asarray(x, dtype=np.float64)
+    Wh = tent_matrix(x.shape[2], f)
+    Ww = tent_matrix(x.shape[3], f)
+    return np.einsum("Ii,bcij,Jj->bcIJ", Wh, x, Ww)
+
+
+def prolong_tent(y, f, H, W):
+    y = np.asarray(y, dtype=np.float64)
+    return np.einsum("Ii,bcIJ,Jj->bcij", tent_matrix(H, f), y, tent_matrix(W, f))
+
+
+class TentLevelsNet(LevelsNet):
+    """LevelsNet with full-weighting restriction and bilinear prolongation."""
+
+    def _levels(self, u):
+        us = [u]
+        for st in self.steps[1:]:
+            us.append(restrict_tent(us[-1], st))
+        return us
+
+    def _ktk(self, us):
+        gs = [conv3x3(conv3x3(ul, w, 1), adjoint_weights(w), 1) for ul, w in zip(us, self.ws)]
+        acc = gs[-1]
+        for l in range(len(gs) - 1, 0, -1):
+            H, W = gs[l - 1].shape[-2:]
+            acc = gs[l - 1] + prolong_tent(acc, self.steps[l], H, W)
+        return acc

@@ -100,11 +100,13 @@ def _net_and_data(arch, spec, B, H, W, seed=0):
     s[:, :, :3, :5] = 0.0
     m = (rng.uniform(size=shp) > 0.3).astype(np.float32)
     ws64 = [w.astype(np.float64) for w in ws]
-    net = OL.Net(ws64, spec, la) if arch == "chain" else OL.LevelsNet(ws64, spec, la)
+    cls = {"chain": OL.Net, "levels": OL.LevelsNet, "tent": OL.TentLevelsNet}[arch]
+    net = cls(ws64, spec, la)
     return p, net, r, e, s, m
 
 
-ARCHS = [("chain", (1, 1, 1, 1)), ("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64)), ("levels", (1, 3, 6))]
+ARCHS = [("chain", (1, 1, 1, 1)), ("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64)), ("levels", (1, 3, 6)),
+         ("tent", (1, 4, 16, 64)), ("tent", (1, 3, 6))]
 
 
 @pytest.mark.parametrize("arch,spec", ARCHS)
@@ -129,7 +131,7 @@ def test_forward_backward_vs_oracle(env, arch, spec):
     assert abs(alpha * tr.dalpha() - dla) < 1e-4 * abs(dla)
 
 
-@pytest.mark.parametrize("arch,spec", [("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64))])
+@pytest.mark.parametrize("arch,spec", [("chain", (1, 3, 9, 27)), ("levels", (1, 4, 16, 64)), ("tent", (1, 8, 32))])
 def test_precon_apply_vs_oracle_and_symmetric(env, arch, spec):
     torch, device, mkgpu = env
     from minkasi_amd.learned.precon import LearnedPrecon
@@ -159,19 +161,21 @@ def test_precon_apply_vs_oracle_and_symmetric(env, arch, spec):
     assert xmx > 0
 
 
-def test_restrict_prolong(env):
+@pytest.mark.parametrize("kind,f", [("box", 4), ("tent", 4), ("tent", 3), ("tent", 8)])
+def test_restrict_prolong(env, kind, f):
     torch, device, mkgpu = env
-    rng = np.random.default_rng(13)
-    B, H, W, f = 2, 37, 45, 4
+    rng = np.random.default_rng(13 + f)
+    B, H, W = 2, 37, 45
     x = rng.standard_normal((B, 1, H, W)).astype(np.float32)
     y = rng.standard_normal((B, 1, -(-H // f), -(-W // f))).astype(np.float32)
     xd, yd = _t(env, x), _t(env, y)
     ry = torch.empty_like(yd)
-    mkgpu.check("mk_restrict_box_f32", xd.data_ptr(), B, H, W, f, ry.data_ptr(), device.stream_ptr())
-    assert rel(ry.cpu().numpy(), OL.restrict(x, f)) < 1e-6
+    R, Pr = (OL.restrict, OL.prolong) if kind == "box" else (OL.restrict_tent, OL.prolong_tent)
+    mkgpu.check("mk_restrict_%s_f32" % kind, xd.data_ptr(), B, H, W, f, ry.data_ptr(), device.stream_ptr())
+    assert rel(ry.cpu().numpy(), R(x, f)) < 1e-6
     px = torch.full_like(xd, 2.0)
-    mkgpu.check("mk_prolong_box_f32", yd.data_ptr(), B, H, W, f, px.data_ptr(), 1, device.stream_ptr())
-    assert rel(px.cpu().numpy(), 2.0 + OL.prolong(y, f, H, W)) < 1e-6
+    mkgpu.check("mk_prolong_%s_f32" % kind, yd.data_ptr(), B, H, W, f, px.data_ptr(), 1, device.stream_ptr())
+    assert rel(px.cpu().numpy(), 2.0 + Pr(y, f, H, W)) < 1e-6
 
 
 def test_adam_matches_numpy(env):

@@ -198,3 +198,39 @@ def test_levels_grad_finite_differences_and_spd():
         M[:, k] = net.apply(v.reshape(H, W), s2).ravel()
     assert np.allclose(M, M.T, atol=1e-12 * np.abs(M).max())
     assert np.linalg.eigvalsh(M).min() > 0
+
+
+@pytest.mark.parametrize("f", [2, 3, 8])
+def test_tent_restriction_adjoint_and_partition_of_unity(f):
+    rng = np.random.default_rng(20 + f)
+    H, W = 5 * f + 3, 4 * f
+    x = rng.standard_normal((1, 1, H, W))
+    y = rng.standard_normal((1, 1, -(-H // f), -(-W // f)))
+    assert abs(np.sum(L.restrict_tent(x, f) * y) - np.sum(x * L.prolong_tent(y, f, H, W))) < 1e-12
+    # interior coarse points average a constant to itself (weights sum to one)
+    c = L.restrict_tent(np.ones((1, 1, 8 * f, 8 * f)), f)
+    assert np.allclose(c[0, 0, 2:-2, 2:-2], 1.0)
+    # bilinear prolongation of a constant is constant in the interior (up to the 1/f^2 scale)
+    p = L.prolong_tent(np.ones((1, 1, 8, 8)), f, 8 * f, 8 * f)
+    assert np.allclose(p[0, 0, 2 * f:-2 * f, 2 * f:-2 * f], 1.0 / (f * f))
+
+
+def test_tent_levels_grad_finite_differences():
+    rng = np.random.default_rng(21)
+    ws = [rng.standard_normal((3, 1, 9)) * 0.4 for _ in range(3)]
+    net = L.TentLevelsNet(ws, (1, 2, 6), log_alpha=-0.1)
+    shp = (1, 1, 15, 13)
+    r, e = rng.standard_normal(shp), rng.standard_normal(shp)
+    s = rng.uniform(0.2, 1.5, shp)
+    m = rng.uniform(size=shp)
+    loss, grads, dla = net.loss_and_grad(r, e, s, m)
+    h = 1e-6
+    for l in range(3):
+        idx = (1, 0, 4)
+        old = net.ws[l][idx]
+        net.ws[l][idx] = old + h
+        lp = net.loss_and_grad(r, e, s, m)[0]
+        net.ws[l][idx] = old - h
+        lm = net.loss_and_grad(r, e, s, m)[0]
+        net.ws[l][idx] = old
+        assert abs((lp - lm) / (2 * h) - grads[l][idx]) < 1e-6 * max(1.0, abs(grads[l][idx]))
