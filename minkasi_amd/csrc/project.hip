@@ -209,7 +209,49 @@ __global__ void __launch_bounds__(256) t2m_groupsort_kernel(const T2MItem* __res
   }
 }
 
-// Exec stage 1: one workgroup (4 waves) per item.
+// Lane-to-lane moves inside a wave through DPP (no LDS round trip).  CTRL is
+// the DPP control word (row_shr:n = 0x110 + n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143); rows outside ROW_MASK and lanes whose source lies
+// outside their row read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Inclusive segmented sum over the wave; d = lane - (first lane of the
+// lane's run), runs contiguous.  Kogge-Stone inside each 16-lane row, then
+// the row-15 and row-31 broadcasts carry a run across row boundaries: after
+// the row steps lane l holds the sum over lanes max(start, row start)..l,
+// and a run that began in an earlier row adds that row's last lane, which
+// holds the earlier part of the same run.
+template <typename T>
+__device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
+  const int dr = min(d, lane & 15);
+  T u;
+  u = dpp_move<0x111, 0xf>(v); v = dr >= 1 ? v + u : v;
+  u = dpp_move<0x112, 0xf>(v); v = dr >= 2 ? v + u : v;
+  u = dpp_move<0x114, 0xf>(v); v = dr >= 4 ? v + u : v;
+  u = dpp_move<0x118, 0xf>(v); v = dr >= 8 ? v + u : v;
+  u = dpp_move<0x142, 0xa>(v); v = ((lane & 16) && d > (lane & 15)) ? v + u : v;  // rows 1, 3
+  u = dpp_move<0x143, 0xc>(v); v = (lane >= 32 && d > lane - 32) ? v + u : v;    // rows 2, 3
+  return v;
+}
+
+// Exec stage 1: one workgroup (4 waves) per item.  Wave w takes the item's
+// 64-entry groups w, w+4, w+8, ... in that order, kT2MUnroll of them per
+// iteration so that their loads overlap.  Within a group the entries are
+// ordered by local pixel, so every pixel is one run; the run's last lane
+// adds the run sum into the wave's private LDS tile (one add per pixel per
+// group, in group order: the result does not depend on timing).
+constexpr int kT2MUnroll = 4;
+
 template <typename T, typename P>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
                                                         const uint32_t* __restrict__ entry,
@@ -226,21 +268,38 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
   const T* base = dat + it.seg_base;
   const int64_t end = it.begin + it.count;
   const int ngroups = (it.count + 63) >> 6;
-  for (int g = wave; g < ngroups; g += 4) {
-    int64_t pos = it.begin + (int64_t)g * 64 + lane;
-    bool valid = pos < end;
-    int lp = valid ? (int)lpix[pos] : -1 - lane;  // distinct sentinels never match
-    T v = valid ? base[entry[pos]] : T(0);
-    // inclusive segmented scan over runs of equal lp (runs are contiguous)
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+  for (int g0 = wave; g0 < ngroups; g0 += 4 * kT2MUnroll) {
+    int lp[kT2MUnroll];
+    uint32_t e[kT2MUnroll];
+    bool valid[kT2MUnroll];
+    T v[kT2MUnroll];
+    // Loads are unconditional (index clamped to the item's last entry, the
+    // item is never empty) so that all of them are in flight together.
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      T u = __shfl_up(v, off, 64);
-      int q = __shfl_up(lp, off, 64);
-      if (lane >= off && q == lp) v += u;
+    for (int j = 0; j < kT2MUnroll; j++) {
+      const int64_t pos = it.begin + (int64_t)(g0 + 4 * j) * 64 + lane;
+      valid[j] = pos < end;  // false for every lane of a group past the item
+      const int64_t pc = valid[j] ? pos : end - 1;
+      lp[j] = (int)lpix[pc];
+      e[j] = entry[pc];
     }
-    int nxt = __shfl_down(lp, 1, 64);
-    bool tail = valid && (lane == 63 || nxt != lp);
-    if (tail) mine[lp] += v;
+#pragma unroll
+    for (int j = 0; j < kT2MUnroll; j++) {
+      const T x = base[e[j]];
+      v[j] = valid[j] ? x : T(0);
+      lp[j] = valid[j] ? lp[j] : -1 - lane;  // distinct sentinels never match
+    }
+#pragma unroll
+    for (int j = 0; j < kT2MUnroll; j++) {
+      if (g0 + 4 * j >= ngroups) break;  // wave-uniform
+      const int prev = __shfl_up(lp[j], 1, 64);
+      const uint64_t heads = __ballot(lane == 0 || prev != lp[j]);
+      const int start = 63 - __clzll(heads & upto);
+      const T s = wave_segmented_sum(v[j], lane, lane - start);
+      const bool tail = valid[j] && (lane == 63 || ((heads >> lane) >> 1) & 1ull);
+      if (tail) __hip_atomic_fetch_add(mine + lp[j], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
   }
   __syncthreads();
   P* out = partial + (int64_t)blockIdx.x * tile_px;
@@ -253,28 +312,52 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
   }
 }
 
-// Exec stage 2: one workgroup per map tile (all tiles, touched or not).
+// Exec stage 2: sums each map tile's partial tiles in item order.  A
+// thread owns 4 consecutive pixels of one tile (tile_px % 4 == 0; a 4-pixel
+// run never crosses a tile row since the plan requires tile_y % 4 == 0)
+// and reads 4 items' partials per step before
+// adding them in order.
+template <typename P>
+struct Vec4;
+template <>
+struct alignas(16) Vec4<float> { float v[4]; };
+template <>
+struct alignas(16) Vec4<double> { double v[4]; };
+
 template <typename T, typename P>
 __global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restrict__ tile_items,
                                                         const P* __restrict__ partial, int tile_x, int tile_y,
                                                         int ntile_y, int nx, int ny, T* __restrict__ map,
-                                                        int accumulate) {
-  const int tile = blockIdx.x;
+                                                        int accumulate, int blocks_per_tile) {
+  const int tile = blockIdx.x / blocks_per_tile;
+  const int tile_px = tile_x * tile_y;
+  const int p0 = ((blockIdx.x % blocks_per_tile) * 256 + threadIdx.x) * 4;
+  if (p0 >= tile_px) return;
   const int tx0 = (tile / ntile_y) * tile_x, ty0 = (tile % ntile_y) * tile_y;
   const int i0 = tile_items[tile], i1 = tile_items[tile + 1];
-  const int tile_px = tile_x * tile_y;
-  for (int p = threadIdx.x; p < tile_px; p += blockDim.x) {
-    int x = tx0 + p / tile_y, y = ty0 + p % tile_y;
-    if (x >= nx || y >= ny) continue;
-    T s = accumulate ? map[(int64_t)x * ny + y] : T(0);
-    T acc = T(0);
-    bool any = false;
-    for (int i = i0; i < i1; i++) {
-      T v = (T)partial[(int64_t)i * tile_px + p];
-      acc = any ? acc + v : v;
-      any = true;
-    }
-    map[(int64_t)x * ny + y] = any ? s + acc : s;
+  const Vec4<P>* src = reinterpret_cast<const Vec4<P>*>(partial) + p0 / 4;
+  const int64_t step = tile_px / 4;  // Vec4 per partial tile
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+  int i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    Vec4<P> a = src[(int64_t)i * step], b = src[(int64_t)(i + 1) * step];
+    Vec4<P> c = src[(int64_t)(i + 2) * step], d = src[(int64_t)(i + 3) * step];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = (((acc[k] + (T)a.v[k]) + (T)b.v[k]) + (T)c.v[k]) + (T)d.v[k];
+  }
+  for (; i < i1; i++) {
+    Vec4<P> a = src[(int64_t)i * step];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += (T)a.v[k];
+  }
+  const int x = tx0 + p0 / tile_y, y0 = ty0 + p0 % tile_y;
+  if (x >= nx) return;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int y = y0 + k;
+    if (y >= ny) break;
+    T* m = map + (int64_t)x * ny + y;
+    *m = accumulate ? *m + acc[k] : acc[k];
   }
 }
 
@@ -309,8 +392,8 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   *out = nullptr;
   MK_REQUIRE(nsamp >= 0 && (nsamp == 0 || ipix), MK_ERR_ARG, "bad ipix/nsamp");
   MK_REQUIRE(nx > 0 && ny > 0 && (int64_t)nx * ny <= INT32_MAX, MK_ERR_ARG, "bad map size");
-  MK_REQUIRE(tile_x > 0 && tile_y > 0 && tile_x * tile_y <= 8192, MK_ERR_ARG,
-             "tile must have 1..8192 pixels (got %d x %d)", tile_x, tile_y);
+  MK_REQUIRE(tile_x > 0 && tile_y > 0 && tile_x * tile_y <= 8192 && tile_y % 4 == 0, MK_ERR_ARG,
+             "tile must have 4..8192 pixels with tile_y a multiple of 4 (got %d x %d)", tile_x, tile_y);
   MK_REQUIRE(item_samples >= 64 && item_samples % 64 == 0, MK_ERR_ARG, "item_samples must be a positive multiple of 64");
   hipStream_t stream = mk::as_stream(stream_v);
   mk_t2m_plan* pl = new mk_t2m_plan();
@@ -448,14 +531,16 @@ int t2m_exec(const mk_t2m_plan* pl, T* map, const T* dat, int accumulate, void* 
     int st = mk::check_launch("t2m_items_kernel");
     if (st) return st;
   }
+  const int bpt = (int)mk::cdiv(tile_px / 4, 256);
+  const unsigned nblk = (unsigned)(ntiles * bpt);
   if (sizeof(T) == 8 && pl->partial_elem_bytes == 8)
-    t2m_tiles_kernel<T, double><<<(unsigned)ntiles, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial,
-                                                                      pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx,
-                                                                      pl->ny, map, accumulate);
+    t2m_tiles_kernel<T, double><<<nblk, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial, pl->tile_x,
+                                                          pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map, accumulate,
+                                                          bpt);
   else
-    t2m_tiles_kernel<T, float><<<(unsigned)ntiles, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial,
-                                                                     pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx,
-                                                                     pl->ny, map, accumulate);
+    t2m_tiles_kernel<T, float><<<nblk, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial, pl->tile_x,
+                                                         pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map, accumulate,
+                                                         bpt);
   return mk::check_launch("t2m_tiles_kernel");
 }
 }  // namespace
