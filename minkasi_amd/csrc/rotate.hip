@@ -59,7 +59,23 @@ __device__ inline void split3_bf16(float x, __bf16* h, __bf16* m, __bf16* l) {
   *l = (__bf16)(r - (float)mm);
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Buffer resource over [p, p + bytes) (raw buffer, gfx9 data format word).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
 // NP = number of products: 3 (mode 0) or 6 (mode 1).
+//
+// Staging of one K step (32 rows of B, 32 columns of A): thread t owns B
+// column n0 + (t % 128) (clamped to N-1: columns past N only feed output
+// columns that are never stored) and the two 8-row strips t/128 and
+// t/128 + 2, and A row t/2, half t%2.  Its byte offsets are fixed per tile;
+// the row and K-step parts are uniform and travel as the buffer instructions'
+// scalar offsets.  The next step's loads are issued before the MFMAs of the
+// current one.  The plan guarantees that every segment's B, C and A planes
+// span less than 4 GiB, so 32-bit offsets hold.
 template <int NP>
 __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict__ segs, int nseg, int64_t ntiles,
                                                        const float* __restrict__ Bbase, float* __restrict__ Cbase) {
@@ -78,10 +94,75 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
   const int64_t local = logical - sg.base;
   const int mb = (int)(local % sg.mblocks), nb = (int)(local / sg.mblocks);
   const int m0 = mb * BM, n0 = nb * BN;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
-  const float* B = Bbase + sg.b_off;
-  float* C = Cbase + sg.c_off;
+  const int K = sg.K;
+
+  const uint32_t ld = (uint32_t)sg.b_ld;
+  const __amdgpu_buffer_rsrc_t brs = buffer_rsrc(Bbase + sg.b_off, (uint32_t)K * ld * 4u);
+  const uint32_t abytes = (uint32_t)(sg.mblocks * BM) * (uint32_t)sg.lda * 2u;
+  __amdgpu_buffer_rsrc_t ars[NPL];
+  ars[0] = buffer_rsrc(sg.a_hi, abytes);
+  ars[1] = buffer_rsrc(sg.a_mid, abytes);
+  if (NPL == 3) ars[NPL - 1] = buffer_rsrc(sg.a_lo, abytes);
+
+  const int nn = tid & (BN - 1), kg0 = tid >> 7;
+  const int gn = min(n0 + nn, sg.N - 1);
+  const uint32_t bvo = ((uint32_t)(kg0 * 8) * ld + (uint32_t)gn) * 4u;
+  const int arow = tid >> 1, ahalf = tid & 1;
+  const uint32_t avo = ((uint32_t)(m0 + arow) * (uint32_t)sg.lda + (uint32_t)(ahalf * 16)) * 2u;
+
+  float xv[2][8];
+  u32x4 av[NPL][2];
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int pl = 0; pl < NPL; pl++) {
+      av[pl][0] = __builtin_amdgcn_raw_buffer_load_b128(ars[pl], avo, (uint32_t)k0 * 2u, 0);
+      av[pl][1] = __builtin_amdgcn_raw_buffer_load_b128(ars[pl], avo + 16u, (uint32_t)k0 * 2u, 0);
+    }
+    if (k0 + BK <= K) {
+#pragma unroll
+      for (int v = 0; v < 2; v++)
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+          xv[v][kk] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(brs, bvo, (uint32_t)(k0 + 16 * v + kk) * ld * 4u, 0));
+    } else {  // last, partial step: rows past K are zero
+#pragma unroll
+      for (int v = 0; v < 2; v++)
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          const int k = k0 + kg0 * 8 + 16 * v + kk;
+          const uint32_t off = ((uint32_t)min(k, K - 1) * ld + (uint32_t)gn) * 4u;
+          const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(brs, off, 0u, 0));
+          xv[v][kk] = k < K ? x : 0.f;
+        }
+    }
+  };
+  auto store_stage = [&]() {
+#pragma unroll
+    for (int pl = 0; pl < NPL; pl++) {
+      *reinterpret_cast<u32x4*>(&As[pl][arow][ahalf * 16]) = av[pl][0];
+      *reinterpret_cast<u32x4*>(&As[pl][arow][ahalf * 16 + 8]) = av[pl][1];
+    }
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      bf16x8 ph, pm, plo;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        __bf16 h, m, l;
+        split3_bf16(xv[v][kk], &h, &m, &l);
+        ph[kk] = h;
+        pm[kk] = m;
+        plo[kk] = l;
+      }
+      const int kg = kg0 + 2 * v;
+      *reinterpret_cast<bf16x8*>(&Bs[0][nn][kg * 8]) = ph;
+      *reinterpret_cast<bf16x8*>(&Bs[1][nn][kg * 8]) = pm;
+      if (NPL == 3) *reinterpret_cast<bf16x8*>(&Bs[NPL - 1][nn][kg * 8]) = plo;
+    }
+  };
 
   f32x16 acc[2][2];
 #pragma unroll
@@ -91,57 +172,11 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
-  for (int k0 = 0; k0 < sg.K; k0 += BK) {
-    // A tile: 128 rows x 32 bf16 (padded storage: no bounds needed)
-    {
-      int row = tid >> 1, half = tid & 1;
-      const __bf16* planes[3] = {sg.a_hi, sg.a_mid, sg.a_lo};
-#pragma unroll
-      for (int pl = 0; pl < NPL; pl++) {
-        const uint4* src = reinterpret_cast<const uint4*>(planes[pl] + (int64_t)(m0 + row) * sg.lda + k0 + half * 16);
-        uint4 v0 = src[0], v1 = src[1];
-        uint4* dst = reinterpret_cast<uint4*>(&As[pl][row][half * 16]);
-        dst[0] = v0;
-        dst[1] = v1;
-      }
-    }
-    // B tile: 32 rows (k) x 128 columns (n), FP32 -> bf16 pieces, stored [n][k].
-    // Each thread takes two (column, 8-k strip) items: the lanes of a load
-    // read consecutive columns of one row (coalesced), and each piece of a
-    // strip goes to LDS as one 16-byte store.  All 16 loads are issued first.
-    {
-      constexpr int ITEMS = BN * (BK / 8), PER = ITEMS / 256;
-      float xv[PER][8];
-#pragma unroll
-      for (int v = 0; v < PER; v++) {
-        const int it = threadIdx.x + v * 256;
-        const int nn = it & (BN - 1), kg = it >> 7;
-        const int gn = n0 + nn;
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) {
-          const int gk = k0 + kg * 8 + kk;
-          xv[v][kk] = (gk < sg.K && gn < sg.N) ? B[(int64_t)gk * sg.b_ld + gn] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < PER; v++) {
-        const int it = threadIdx.x + v * 256;
-        const int nn = it & (BN - 1), kg = it >> 7;
-        bf16x8 ph, pm, plo;
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) {
-          __bf16 h, m, l;
-          split3_bf16(xv[v][kk], &h, &m, &l);
-          ph[kk] = h;
-          pm[kk] = m;
-          plo[kk] = l;
-        }
-        *reinterpret_cast<bf16x8*>(&Bs[0][nn][kg * 8]) = ph;
-        *reinterpret_cast<bf16x8*>(&Bs[1][nn][kg * 8]) = pm;
-        if (NPL == 3) *reinterpret_cast<bf16x8*>(&Bs[NPL - 1][nn][kg * 8]) = plo;
-      }
-    }
+  load_stage(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store_stage();
     __syncthreads();
+    if (k0 + BK < K) load_stage(k0 + BK);  // in flight during the MFMAs below
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
       const int kq = ks + 8 * (lane >> 5);
@@ -173,16 +208,21 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
     __syncthreads();
   }
   // epilogue: D[row][col], row = 8*(r/4) + 4*(lane/32) + r%4, col = lane%32
+  // (Plain global stores: raw buffer stores of accumulator elements were
+  // compiled to store element 0 of each block for every r.)
+  float* C = Cbase + sg.c_off;
+  const uint32_t cld = (uint32_t)sg.c_ld;
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-      int col = n0 + wn * 64 + j * 32 + (lane & 31);
+      const int col = n0 + wn * 64 + j * 32 + (lane & 31);
       if (col >= sg.N) continue;
+      const uint32_t cvo = (uint32_t)(4 * (lane >> 5)) * cld + (uint32_t)col;
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (row < sg.M) C[(int64_t)row * sg.c_ld + col] = acc[i][j][r];
+        const int row_u = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + (r & 3);  // uniform part of the row
+        if (row_u + 4 * (lane >> 5) < sg.M) C[(uint32_t)row_u * cld + cvo] = acc[i][j][r];
       }
     }
 }
@@ -352,6 +392,11 @@ extern "C" int mk_rot_plan_create(int nseg, const int64_t* desc, int mode, mk_ro
                MK_ERR_ARG, "segment %d: bad sizes", i);
     MK_REQUIRE(s.a_hi && s.a_mid && s.a_lo, MK_ERR_ARG, "segment %d: null A", i);
     MK_REQUIRE(s.lda % BK == 0 && s.lda >= s.K, MK_ERR_ARG, "segment %d: lda must be a multiple of %d >= K", i, BK);
+    MK_REQUIRE(s.b_ld >= s.N && s.c_ld >= s.N && s.b_off >= 0 && s.c_off >= 0, MK_ERR_ARG,
+               "segment %d: row strides must cover N and offsets be >= 0", i);
+    MK_REQUIRE((int64_t)s.K * s.b_ld * 4 <= UINT32_MAX && (int64_t)s.M * s.c_ld * 4 <= UINT32_MAX &&
+                   mk::cdiv(s.M, BM) * BM * s.lda * 2 <= UINT32_MAX,
+               MK_ERR_ARG, "segment %d: B, C and A planes must each span less than 4 GiB", i);
     MK_REQUIRE(((uintptr_t)s.a_hi & 15) == 0 && ((uintptr_t)s.a_mid & 15) == 0 && ((uintptr_t)s.a_lo & 15) == 0,
                MK_ERR_ARG, "A must be 16-B aligned");
     s.mblocks = (int32_t)mk::cdiv(s.M, BM);
