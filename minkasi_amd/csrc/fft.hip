@@ -85,7 +85,9 @@ constexpr int kMaxLds = 144 * 1024;            // dynamic LDS budget per workgro
 #define MK_FFT_PAD_SHIFT 4
 #endif
 constexpr __host__ __device__ __forceinline__ int pad8(int e) { return e + (e >> MK_FFT_PAD_SHIFT); }
-constexpr __host__ __device__ __forceinline__ int row_stride(int len) { return len + (len >> MK_FFT_PAD_SHIFT) + 1; }
+// Odd, so that rows read in parallel fall in different banks (for a power of
+// two len this is len + len/16 + 1).
+constexpr __host__ __device__ __forceinline__ int row_stride(int len) { return (len + (len >> MK_FFT_PAD_SHIFT)) | 1; }
 
 // In-register DFTs (natural order in and out).
 template <typename T, bool INV>
@@ -219,6 +221,134 @@ __device__ void fill_stage_table(cx<T>* tab, int l2len) {
 template <typename T>
 __device__ __forceinline__ cx<T> twiddle_P(int64_t a, T invP) { return expmi2<T>(2 * a, invP); }
 
+// ---- Mixed radix (2, 3, 5) transform lengths ---------------------------------
+// Radix of the next stage of an N-point Stockham FFT when rem = N / Ns points
+// per sub-transform remain to be combined: the power-of-two part first (8
+// while 8 divides rem, then one 4 or 2; for N = 2^q this is the stage_lr
+// sequence), then 3s, then 5s.  Stage tables are laid out in this order.
+constexpr __host__ __device__ int next_radix(int rem) {
+  return (rem & 1) == 0 ? ((rem & 7) == 0 ? 8 : ((rem & 3) == 0 ? 4 : 2)) : (rem % 3 == 0 ? 3 : 5);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(cx<T>* v) {
+  const T h = T(0.86602540378443864676);  // sin(2 pi / 3)
+  cx<T> a = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  cx<T> t = {v[0].re - T(0.5) * a.re, v[0].im - T(0.5) * a.im};
+  cx<T> u = INV ? cx<T>{-h * d.im, h * d.re} : cx<T>{h * d.im, -h * d.re};  // (+/-i) sin(2pi/3) d
+  v[0] = cadd(v[0], a);
+  v[1] = cadd(t, u);
+  v[2] = csub(t, u);
+}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft5(cx<T>* v) {
+  const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);  // cos(2pi/5), cos(4pi/5)
+  const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);   // sin(2pi/5), sin(4pi/5)
+  cx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]), a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  cx<T> t1 = {v[0].re + c1 * a1.re + c2 * a2.re, v[0].im + c1 * a1.im + c2 * a2.im};
+  cx<T> t2 = {v[0].re + c2 * a1.re + c1 * a2.re, v[0].im + c2 * a1.im + c1 * a2.im};
+  cx<T> u1 = {s1 * b1.re + s2 * b2.re, s1 * b1.im + s2 * b2.im};
+  cx<T> u2 = {s2 * b1.re - s1 * b2.re, s2 * b1.im - s1 * b2.im};
+  // forward: y1 = t1 - i u1, y4 = t1 + i u1, y2 = t2 - i u2, y3 = t2 + i u2
+  cx<T> iu1 = INV ? cx<T>{-u1.im, u1.re} : cx<T>{u1.im, -u1.re};
+  cx<T> iu2 = INV ? cx<T>{-u2.im, u2.re} : cx<T>{u2.im, -u2.re};
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(t1, iu1);
+  v[4] = csub(t1, iu1);
+  v[2] = cadd(t2, iu2);
+  v[3] = csub(t2, iu2);
+}
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void dft_r(cx<T>* v) {
+  if constexpr (R == 8) dft8<T, INV>(v);
+  else if constexpr (R == 4) dft4v<T, INV>(v);
+  else if constexpr (R == 2) dft2<T, INV>(v);
+  else if constexpr (R == 3) dft3<T, INV>(v);
+  else dft5<T, INV>(v);
+}
+
+// One radix-R stage of an N-point mixed-radix Stockham FFT over nrows padded
+// rows (the runtime counterpart of fft_stage): butterfly j of a row reads
+// j + r N/R, twiddles by tw[(r-1) Ns + k] (k = j mod Ns) and writes
+// (j / Ns) Ns R + k + r Ns.
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void fft_stage_mx(cx<T>* buf, const cx<T>* tw, int N, int nrows, int rs, int Ns) {
+  constexpr int NB = (kMaxWG / R + kThreads - 1) / kThreads;
+  const int q = N / R, nbf = nrows * q;
+  cx<T> v[NB][R];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    if (bf < nbf) {
+      const int row = bf / q, j = bf - row * q, k = j % Ns;
+      const cx<T>* base = buf + row * rs;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * q)];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          cx<T> w = tw[(r - 1) * Ns + k];
+          if (INV) w = cconj(w);
+          v[u][r] = cmul(v[u][r], w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    if (bf < nbf) {
+      const int row = bf / q, j = bf - row * q, k = j % Ns;
+      dft_r<T, INV, R>(v[u]);
+      cx<T>* base = buf + row * rs;
+      const int idx = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) base[pad8(idx + r * Ns)] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place N-point FFT (N 5-smooth) of nrows padded rows, nrows * N <= kMaxWG;
+// tw = fill_stage_table_mx table.
+template <typename T, bool INV>
+__device__ void lds_fft_mx(cx<T>* buf, const cx<T>* tw, int N, int nrows, int rs) {
+  int Ns = 1, off = 0;
+  while (Ns < N) {
+    const int R = next_radix(N / Ns);
+    switch (R) {
+      case 8: fft_stage_mx<T, INV, 8>(buf, tw + off, N, nrows, rs, Ns); break;
+      case 4: fft_stage_mx<T, INV, 4>(buf, tw + off, N, nrows, rs, Ns); break;
+      case 2: fft_stage_mx<T, INV, 2>(buf, tw + off, N, nrows, rs, Ns); break;
+      case 3: fft_stage_mx<T, INV, 3>(buf, tw + off, N, nrows, rs, Ns); break;
+      default: fft_stage_mx<T, INV, 5>(buf, tw + off, N, nrows, rs, Ns); break;
+    }
+    off += (R - 1) * Ns;
+    Ns *= R;
+  }
+}
+
+// Stage table of an N-point mixed-radix FFT (next_radix order): entry
+// (r-1) Ns + k of a stage is exp(-2 pi i k r / (R Ns)); N - 1 entries.
+template <typename T>
+__device__ void fill_stage_table_mx(cx<T>* tab, int N) {
+  int Ns = 1, off = 0;
+  while (Ns < N) {
+    const int R = next_radix(N / Ns), cnt = (R - 1) * Ns;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int r = e / Ns + 1, k = e - (r - 1) * Ns;
+      tab[off + e] = expmi<T>(2 * (int64_t)k * r, (int64_t)R * Ns);
+    }
+    off += cnt;
+    Ns *= R;
+  }
+}
+
+// exp(-2 pi i a / P) for 0 <= a < P and any P (FP64 argument reduction).
+template <typename T>
+__device__ __forceinline__ cx<T> twiddle_any(int64_t a, int64_t P) { return expmi<T>(2 * a, P); }
+
 // Segment: `rows` rows sharing one length n and one plan.
 struct Seg {
   int64_t in_off, in_stride;    // input rows (elements of the input type)
@@ -227,10 +357,10 @@ struct Seg {
   int64_t s_off, s_stride;      // spectrum rows (filter: P+1 reals; Bluestein: P complex, stride 0)
   void* s_ptr;                  // device address of this segment's spectrum rows (set at upload)
   void* tw_ptr;                 // hot K2: fine/coarse twiddle tables of this P (set at create)
-  int32_t rows, n, l1, l2, cw, flags;
+  int32_t rows, n, l1, l2, cw, flags;  // l1 = log2(P1), or -1 when P1 is not a power of two
   int64_t base1, base2, base3;  // first global item of K1, K2, K3
   int32_t per1, per2;           // items per row for K1/K3, K2
-  int32_t r2, pad2;             // k1 values per K2 item
+  int32_t r2, p1;               // k1 values per K2 item; column length P1 (2^a 3^b 5^c)
 };
 
 enum Mode { kFilter = 0, kFSpec = 1, kBlue = 2, kBSpec = 3 };
@@ -256,7 +386,11 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k1_kernel(co
   const Seg sg = segs[s];
   const int64_t local = (int64_t)blockIdx.x - sg.base1;
   const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
-  const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
+  const int l1 = sg.l1, l2 = sg.l2, P1 = sg.p1, P2 = 1 << l2, cw = sg.cw;
+  // Mixed-radix column lengths reach this kernel only in the spectrum pass
+  // (the filter runs them on the hot path), so only that mode carries the
+  // mixed-radix code.
+  const bool pow2 = MODE != kFSpec || l1 >= 0;
   const int lcw = 31 - __clz(cw);
   const int rs = row_stride(P1);
   const int64_t P = (int64_t)P1 * P2;
@@ -265,7 +399,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k1_kernel(co
   cx<T>* tw = buf + cw * rs;                          // [P1] stage table
   const TIn* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * cw;
-  fill_stage_table<T>(tw, l1);
+  if (pow2) fill_stage_table<T>(tw, l1);
+  else if constexpr (MODE == kFSpec) fill_stage_table_mx<T>(tw, P1);
   constexpr int UNR = 8;  // loads in flight per thread
   if (MODE == kFilter || MODE == kFSpec) {
     // real input packed two per complex point: a_(2j), a_(2j+1)
@@ -324,14 +459,15 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k1_kernel(co
     }
   }
   __syncthreads();
-  lds_fft<T, false>(buf, tw, l1, cw, rs);
+  if (pow2) lds_fft<T, false>(buf, tw, l1, cw, rs);
+  else if constexpr (MODE == kFSpec) lds_fft_mx<T, false>(buf, tw, P1, cw, rs);
   cx<T>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   const T invP = T(1) / (T)P;
   for (int idx = threadIdx.x; idx < P1 * cw; idx += kThreads) {
     int k1 = idx >> lcw, c = idx & (cw - 1);
     int64_t j2 = j2_0 + c;
-    int64_t a = (j2 * k1) & (P - 1);
-    cx<T> v = cmul(buf[c * rs + pad8(k1)], twiddle_P<T>(a, invP));
+    int64_t a = (j2 * k1) % P;
+    cx<T> v = cmul(buf[c * rs + pad8(k1)], pow2 ? twiddle_P<T>(a, invP) : twiddle_any<T>(a, P));
     trow[(int64_t)k1 * P2 + j2] = v;
   }
 }
@@ -354,7 +490,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
   const Seg sg = segs[s];
   const int64_t local = (int64_t)blockIdx.x - sg.base2;
   const int row = (int)(local / sg.per2), grp = (int)(local % sg.per2);
-  const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2;
+  const int l2 = sg.l2, P1 = sg.p1, P2 = 1 << l2;
+  const bool pow2 = sg.l1 >= 0;
   const int rs = row_stride(P2);
   const int64_t P = (int64_t)P1 * P2;
   const bool real_mode = (MODE == kFilter || MODE == kFSpec);
@@ -367,7 +504,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
       int a = ns++;
       slot_k1[a] = k1;
       slot_partner[a] = a;
-      int k1b = (P1 - k1) & (P1 - 1);
+      int k1b = k1 == 0 ? 0 : P1 - k1;
       if (real_mode && k1b != k1) {
         int b = ns++;
         slot_k1[b] = k1b;
@@ -442,7 +579,7 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
       cx<T> E = cscale(cadd(C, cconj(Cq)), T(0.5));
       cx<T> Dd = csub(C, cconj(Cq));
       cx<T> O = {Dd.im * T(0.5), -Dd.re * T(0.5)};
-      cx<T> w = expmi2<T>(k, invP);  // w_L^k; w_L^(P-k) = -conj(w)
+      cx<T> w = pow2 ? expmi2<T>(k, invP) : expmi<T>(k, P);  // w_L^k; w_L^(P-k) = -conj(w)
       cx<T> A = cadd(E, cmul(w, O));
       cx<T> Aq = csub(cconj(E), cmul(cconj(w), cconj(O)));
       int pk1 = slot_k1[pidx >> l2], pk2 = pidx & (P2 - 1);
@@ -480,14 +617,15 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k2_kernel(co
   for (int idx = threadIdx.x; idx < nslot * P2; idx += kThreads) {
     int sl = idx >> l2, j2 = idx & (P2 - 1);
     int kk1 = slot_k1[sl];
-    int64_t a = ((int64_t)j2 * kk1) & (P - 1);
-    trow[(int64_t)kk1 * P2 + j2] = cmul(at(idx), cconj(twiddle_P<T>(a, invP2)));
+    int64_t a = ((int64_t)j2 * kk1) % P;
+    trow[(int64_t)kk1 * P2 + j2] = cmul(at(idx), cconj(pow2 ? twiddle_P<T>(a, invP2) : twiddle_any<T>(a, P)));
   }
 }
 
 // ---- K2, FP32 filter hot path: rows of 2^11 points --------------------------
 // Every workgroup holds exactly two rows: k1 and its partner P1-k1, or, for
-// the first workgroup of a mode, the two self-paired rows k1 = 0 and P1/2.
+// the first workgroup of a mode, the self-paired rows k1 = 0 and (even P1)
+// P1/2; with odd P1 the second slot of that workgroup is idle.
 // Stage and four-step twiddles come from tables built once per plan in FP64
 // (no transcendental in the loop); the stage shapes are compile-time, so the
 // LDS addresses fold into instruction offsets.  Each thread owns four
@@ -565,9 +703,10 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
   const Seg sg = segs[s];
   const int local = (int)((int64_t)blockIdx.x - sg.base2);
   const int row = local / sg.per2, g = local - row * sg.per2;
-  const int l1 = sg.l1, P1 = 1 << l1;
+  const int P1 = sg.p1;
   const int P = P1 << kHotL2;
-  const int kA = g == 0 ? 0 : g, kB = g == 0 ? P1 / 2 : P1 - g;
+  const bool single = g == 0 && (P1 & 1);  // odd P1: column 0 is the only self-paired one
+  const int kA = g == 0 ? 0 : g, kB = g == 0 ? (single ? 0 : P1 / 2) : P1 - g;
   cx<float>* X = reinterpret_cast<cx<float>*>(smem_raw);  // [2][kHotRS]
   cx<float>* F = X + 2 * kHotRS;                           // [1024] w_2P^l
   cx<float>* Ct = F + (1 << kFineBits);                    // [4 P1] w_2P^(1024 h)
@@ -606,7 +745,7 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
       if (pl < kHotP2) { sl = 0; k2 = pl; slq = 1; k2q = kHotP2 - 1 - pl; }
     } else if (pl <= kHotP2 / 2) {
       sl = 0; k2 = pl; slq = 0; k2q = (kHotP2 - pl) & (kHotP2 - 1);
-    } else if (pl < kHotP2 + 1) {
+    } else if (!single && pl < kHotP2 + 1) {
       sl = 1; k2 = pl - (kHotP2 / 2 + 1); slq = 1; k2q = kHotP2 - 1 - k2;
     }
     p_sl[u] = sl; p_k2[u] = k2; p_slq[u] = slq; p_k2q[u] = k2q;
@@ -663,35 +802,97 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
   }
   __syncthreads();
   ct_fft<true, kHotL2, 2, kHotRS>(X, st);
-  cx<float> tA = tw2P(F, Ct, 2 * (int)(((int64_t)kA * threadIdx.x) & (P - 1)));
-  cx<float> tB = tw2P(F, Ct, 2 * (int)(((int64_t)kB * threadIdx.x) & (P - 1)));
-  const cx<float> sA = tw2P(F, Ct, 2 * (int)(((int64_t)kA * kThreads) & (P - 1)));
-  const cx<float> sB = tw2P(F, Ct, 2 * (int)(((int64_t)kB * kThreads) & (P - 1)));
+  // kA, kB < 256 and tid < 512, so the products fit in 32 bits
+  const uint32_t uP = (uint32_t)P;
+  cx<float> tA = tw2P(F, Ct, 2 * (int)(((uint32_t)kA * threadIdx.x) % uP));
+  cx<float> tB = tw2P(F, Ct, 2 * (int)(((uint32_t)kB * threadIdx.x) % uP));
+  const cx<float> sA = tw2P(F, Ct, 2 * (int)(((uint32_t)kA * kThreads) % uP));
+  const cx<float> sB = tw2P(F, Ct, 2 * (int)(((uint32_t)kB * kThreads) % uP));
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j2 = threadIdx.x + u * kThreads;
     trow[kA * kHotP2 + j2] = cmul(X[pad8(j2)], cconj(tA));
-    trow[kB * kHotP2 + j2] = cmul(X[kHotRS + pad8(j2)], cconj(tB));
+    if (!single) trow[kB * kHotP2 + j2] = cmul(X[kHotRS + pad8(j2)], cconj(tB));
     tA = cmul(tA, sA);
     tB = cmul(tB, sB);
   }
 }
 
-// ---- K1 / K3, FP32 filter hot path: columns of P1 = 2^L1 (rows of 2^11) ----
-// One launch per column length; CW = 4096 / P1 adjacent columns per
-// workgroup, so global reads and writes are runs of CW points.
-template <int L1>
+// ---- K1 / K3, FP32 filter hot path: columns of P1 = 2^a 3^b 5^c (rows of 2^11)
+// One launch per column length; CW adjacent columns per workgroup (the
+// largest power of two with CW P1 <= 4096), so global reads and writes are
+// runs of CW points.  The column FFT is a compile-time mixed-radix Stockham
+// sequence (next_radix order) over the CW padded rows.
+constexpr int hot_cw(int p1) {
+  int c = 1;
+  while (2 * c * p1 <= kMaxWG && 2 * c <= kHotP2) c *= 2;
+  return c;
+}
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// One radix-R stage of an N-point FFT over NROWS padded rows (stride RS);
+// the tail of the last butterfly round is masked when NROWS N / R is not a
+// multiple of the workgroup size.
+template <bool INV, int R, int N, int NS, int NROWS, int RS>
+__device__ __forceinline__ void ct_stage_mx(cx<float>* buf, const cx<float>* tw) {
+  constexpr int Q = N / R, TOT = NROWS * Q, NB = (TOT + kThreads - 1) / kThreads;
+  constexpr bool FULL = NB * kThreads == TOT;
+  cx<float> v[NB][R];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    if (FULL || bf < TOT) {
+      const int row = bf / Q, j = bf - row * Q, k = j % NS;
+      const cx<float>* base = buf + row * RS;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * Q)];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          cx<float> w = tw[(r - 1) * NS + k];
+          if (INV) w = cconj(w);
+          v[u][r] = cmul(v[u][r], w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int bf = threadIdx.x + u * kThreads;
+    if (FULL || bf < TOT) {
+      const int row = bf / Q, j = bf - row * Q, k = j % NS;
+      dft_r<float, INV, R>(v[u]);
+      cx<float>* base = buf + row * RS;
+      const int idx = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) base[pad8(idx + r * NS)] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <bool INV, int N, int NROWS, int RS, int NS = 1, int OFF = 0>
+__device__ __forceinline__ void ct_fft_mx(cx<float>* buf, const cx<float>* st) {
+  if constexpr (NS < N) {
+    constexpr int R = next_radix(N / NS);
+    ct_stage_mx<INV, R, N, NS, NROWS, RS>(buf, st + OFF);
+    ct_fft_mx<INV, N, NROWS, RS, NS * R, OFF + (R - 1) * NS>(buf, st);
+  }
+}
+
+template <int P1>
 __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restrict__ segs, int nseg,
                                                               const float* __restrict__ in,
                                                               cx<float>* __restrict__ tbuf,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = row_stride(P1);
+  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), RS = row_stride(P1), NPTS = CW * P1;
+  constexpr int P = P1 * kHotP2;
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 1);
   const Seg sg = segs[s];
   const int local = (int)((int64_t)blockIdx.x - sg.base1);
   const int row = local / sg.per1, jb = local - row * sg.per1;
-  const int P = P1 << kHotL2;
   const int m = sg.n - 1;
   cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
   cx<float>* st = buf + CW * RS;                             // [P1] stage table
@@ -704,7 +905,7 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
   const float* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * CW;
   // real input a_i, i = 2 (P2 j1 + j2) + w, from the even extension on [-m, 2m]
-  constexpr int NV = 2 * kMaxWG / kThreads;  // 16 reals per thread
+  constexpr int NR = 2 * NPTS, NV = (NR + kThreads - 1) / kThreads;
   float val[NV];
 #pragma unroll
   for (int u = 0; u < NV; u++) {
@@ -712,7 +913,7 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
     const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
     const int i = 2 * (kHotP2 * j1 + j2_0) + w;
     val[u] = 0.f;
-    if (i <= 3 * m) {
+    if ((NV * kThreads == NR || idx < NR) && i <= 3 * m) {
       const int t = i - m;
       val[u] = x[t < 0 ? -t : (t <= m ? t : 2 * m - t)];
     }
@@ -721,32 +922,34 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
   for (int u = 0; u < NV; u++) {
     const int idx = threadIdx.x + u * kThreads;
     const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
-    reinterpret_cast<float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] = val[u];
+    if (NV * kThreads == NR || idx < NR) reinterpret_cast<float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] = val[u];
   }
   __syncthreads();
-  ct_fft<false, L1, CW, RS>(buf, st);
+  ct_fft_mx<false, P1, CW, RS>(buf, st);
   cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
+  constexpr int NO = (NPTS + kThreads - 1) / kThreads;
 #pragma unroll
-  for (int u = 0; u < kMaxWG / kThreads; u++) {
+  for (int u = 0; u < NO; u++) {
     const int idx = threadIdx.x + u * kThreads;
+    if (NO * kThreads != NPTS && idx >= NPTS) break;
     const int k1 = idx >> LCW, c = idx & (CW - 1);
-    const int a = (int)(((int64_t)(j2_0 + c) * k1) & (P - 1));
+    const int a = ((j2_0 + c) * k1) % P;  // < 2^11 * 2^8
     trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[c * RS + pad8(k1)], tw2P(F, Ct, 2 * a));
   }
 }
 
-template <int L1>
+template <int P1>
 __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restrict__ segs, int nseg,
                                                               const cx<float>* __restrict__ tbuf,
                                                               float* __restrict__ out,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int P1 = 1 << L1, CW = kMaxWG / P1, LCW = 12 - L1, RS = row_stride(P1);
+  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), RS = row_stride(P1), NPTS = CW * P1;
+  constexpr int P = P1 * kHotP2;
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 3);
   const Seg sg = segs[s];
   const int local = (int)((int64_t)blockIdx.x - sg.base3);
   const int row = local / sg.per1, jb = local - row * sg.per1;
-  const int P = P1 << kHotL2;
   const int m = sg.n - 1;
   cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
   cx<float>* st = buf + CW * RS;                             // [P1] stage table
@@ -754,27 +957,29 @@ __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restri
   const cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   const int j2_0 = jb * CW;
   {
-    constexpr int NV = kMaxWG / kThreads;  // 8 points per thread
+    constexpr int NV = (NPTS + kThreads - 1) / kThreads;
     cx<float> v[NV];
 #pragma unroll
     for (int u = 0; u < NV; u++) {
       const int idx = threadIdx.x + u * kThreads;
-      v[u] = trow[(idx >> LCW) * kHotP2 + j2_0 + (idx & (CW - 1))];
+      if (NV * kThreads == NPTS || idx < NPTS) v[u] = trow[(idx >> LCW) * kHotP2 + j2_0 + (idx & (CW - 1))];
     }
 #pragma unroll
     for (int u = 0; u < NV; u++) {
       const int idx = threadIdx.x + u * kThreads;
-      buf[(idx & (CW - 1)) * RS + pad8(idx >> LCW)] = v[u];
+      if (NV * kThreads == NPTS || idx < NPTS) buf[(idx & (CW - 1)) * RS + pad8(idx >> LCW)] = v[u];
     }
   }
   __syncthreads();
-  ct_fft<true, L1, CW, RS>(buf, st);
+  ct_fft_mx<true, P1, CW, RS>(buf, st);
   const float scale = 1.f / (float)P;
   const bool halve = sg.flags & 1;
   float* y = out + sg.out_off + (int64_t)row * sg.out_stride;
+  constexpr int NR = 2 * NPTS, NO = (NR + kThreads - 1) / kThreads;
 #pragma unroll
-  for (int u = 0; u < 2 * kMaxWG / kThreads; u++) {
+  for (int u = 0; u < NO; u++) {
     const int idx = threadIdx.x + u * kThreads;
+    if (NO * kThreads != NR && idx >= NR) break;
     const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
     const int k = 2 * (kHotP2 * j1 + j2_0) + w - m;
     if (k >= 0 && k <= m) {
@@ -785,11 +990,39 @@ __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restri
   }
 }
 
+// Column lengths of the hot path: every 5-smooth P1 in [2, 256].
+constexpr int kHotP1[] = {2,   3,   4,   5,   6,   8,   9,   10,  12,  15,  16,  18,  20,  24,  25,  27,  30,
+                          32,  36,  40,  45,  48,  50,  54,  60,  64,  72,  75,  80,  81,  90,  96,  100, 108,
+                          120, 125, 128, 135, 144, 150, 160, 162, 180, 192, 200, 216, 225, 240, 243, 250, 256};
+constexpr int kNumHotP1 = sizeof(kHotP1) / sizeof(kHotP1[0]);
+int hot_p1_index(int p1) {
+  for (int i = 0; i < kNumHotP1; i++)
+    if (kHotP1[i] == p1) return i;
+  return -1;
+}
+
 // LDS of the hot K1 (tile, stage table, fine/coarse tables) and K3 (tile,
 // stage table).
-size_t hot13_lds(int l1, int which) {
-  const int P1 = 1 << l1, CW = kMaxWG / P1, RS = row_stride(P1);
-  return (size_t)(CW * RS + P1 + (which == 1 ? (1 << kFineBits) + 4 * P1 : 0)) * sizeof(cx<float>);
+size_t hot13_lds(int p1, int which) {
+  const int CW = hot_cw(p1), RS = row_stride(p1);
+  return (size_t)(CW * RS + p1 + (which == 1 ? (1 << kFineBits) + 4 * p1 : 0)) * sizeof(cx<float>);
+}
+
+// Stage table of an N-point mixed-radix FFT (fill_stage_table_mx order),
+// computed in FP64.
+__global__ void mixed_stage_table_kernel(cx<float>* out, int N) {
+  int Ns = 1, off = 0;
+  while (Ns < N) {
+    const int R = next_radix(N / Ns), cnt = (R - 1) * Ns;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+      const int r = e / Ns + 1, k = e - (r - 1) * Ns;
+      double sn, cs;
+      sincospi(-2.0 * (double)(k * r) / (double)(R * Ns), &sn, &cs);
+      out[off + e] = {(float)cs, (float)sn};
+    }
+    off += cnt;
+    Ns *= R;
+  }
 }
 
 // Twiddle tables of the hot K2 path, computed in FP64: the stage table of a
@@ -829,6 +1062,8 @@ __global__ void __launch_bounds__(kThreads, sizeof(T) == 4 ? 4 : 1) k3_kernel(co
   const Seg sg = segs[s];
   const int64_t local = (int64_t)blockIdx.x - sg.base3;
   const int row = (int)(local / sg.per1), jb = (int)(local % sg.per1);
+  // power-of-two column lengths only (the plan routes mixed lengths to the
+  // hot path)
   const int l1 = sg.l1, l2 = sg.l2, P1 = 1 << l1, P2 = 1 << l2, cw = sg.cw;
   const int lcw = 31 - __clz(cw);
   const int rs = row_stride(P1);
@@ -902,7 +1137,8 @@ int log2_ceil(int64_t x) {
 }
 
 struct Geometry {
-  int l1, l2, cw;
+  int l1, l2, cw;  // l1 = log2(p1), or -1 when p1 is not a power of two
+  int p1;
   int64_t P;
 };
 
@@ -911,7 +1147,24 @@ struct Geometry {
 // workgroup), so the columns are short and K1/K3 can take many adjacent
 // columns at once: their global accesses are runs of cw complex points.
 // MK_FFT_L2MAX (power-of-two exponent, 2..11) lowers the row-length cap.
-Geometry geometry(int64_t P_min, int elem_bytes) {
+//
+// mixed: the FP32 hot path takes any 5-smooth P1 in [2, 256] with P2 = 2^11,
+// so P = P1 2^11 is the smallest such multiple of 2^11 >= P_min (instead of
+// the next power of two): about 1.4x fewer points on average.
+Geometry geometry(int64_t P_min, int elem_bytes, bool mixed = false) {
+  if (mixed && P_min > kHotP2) {
+    const int64_t need = (P_min + kHotP2 - 1) / kHotP2;
+    for (int i = 0; i < kNumHotP1; i++)
+      if (kHotP1[i] >= need) {
+        Geometry g;
+        g.p1 = kHotP1[i];
+        g.l1 = (g.p1 & (g.p1 - 1)) == 0 ? ilog2c(g.p1) : -1;
+        g.l2 = kHotL2;
+        g.P = (int64_t)g.p1 * kHotP2;
+        g.cw = hot_cw(g.p1);
+        return g;
+      }
+  }
   int q = std::max(2, log2_ceil(P_min));
   static int l2max = [] {
     const char* e = getenv("MK_FFT_L2MAX");
@@ -921,8 +1174,9 @@ Geometry geometry(int64_t P_min, int elem_bytes) {
   Geometry g;
   g.l2 = std::min(l2max, q - 1);
   g.l1 = q - g.l2;
+  g.p1 = 1 << g.l1;
   g.P = int64_t(1) << q;
-  int P1 = 1 << g.l1, P2 = 1 << g.l2;
+  int P1 = g.p1, P2 = 1 << g.l2;
   int cw = std::min(std::max(1, kMaxWG / P1), P2);
   // keep the K1/K3 tile plus the twiddle table within the LDS budget
   while (cw > 1 && ((int64_t)cw * row_stride(P1) + P1) * 2 * elem_bytes > kMaxLds) cw >>= 1;
@@ -934,6 +1188,15 @@ template <typename K>
 void allow_big_lds(K kernel) {
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                             kMaxLds);
+}
+
+template <int I = 0>
+void allow_lds_hot13() {
+  if constexpr (I < kNumHotP1) {
+    allow_big_lds(k1_hot_kernel<kHotP1[I]>);
+    allow_big_lds(k3_hot_kernel<kHotP1[I]>);
+    allow_lds_hot13<I + 1>();
+  }
 }
 
 void allow_lds_all() {
@@ -962,17 +1225,12 @@ void allow_lds_all() {
   allow_big_lds(k3_kernel<float, float, kBlue>);
   allow_big_lds(k3_kernel<double, double, kBlue>);
   allow_big_lds(k2_hot_kernel);
-  allow_big_lds(k1_hot_kernel<1>); allow_big_lds(k1_hot_kernel<2>); allow_big_lds(k1_hot_kernel<3>);
-  allow_big_lds(k1_hot_kernel<4>); allow_big_lds(k1_hot_kernel<5>); allow_big_lds(k1_hot_kernel<6>);
-  allow_big_lds(k1_hot_kernel<7>); allow_big_lds(k1_hot_kernel<8>);
-  allow_big_lds(k3_hot_kernel<1>); allow_big_lds(k3_hot_kernel<2>); allow_big_lds(k3_hot_kernel<3>);
-  allow_big_lds(k3_hot_kernel<4>); allow_big_lds(k3_hot_kernel<5>); allow_big_lds(k3_hot_kernel<6>);
-  allow_big_lds(k3_hot_kernel<7>); allow_big_lds(k3_hot_kernel<8>);
+  allow_lds_hot13();
   (void)hipGetLastError();  // an unsupported attribute must not poison later launch checks
 }
 
 size_t lds_k13(const Geometry& g, int eb) {
-  return (size_t)(g.cw * row_stride(1 << g.l1) + (1 << g.l1)) * 2 * eb;
+  return (size_t)(g.cw * row_stride(g.p1) + g.p1) * 2 * eb;
 }
 // k1 values per K2 workgroup: slots (2 per k1 for real modes) * P2 <= kMaxWG
 // LDS of K2: slot rows, the stage table and (real modes) the slots' spectrum
@@ -983,7 +1241,7 @@ size_t k2_bytes(int r, int P2, int eb, bool real) {
 }
 // k1 values per K2 workgroup: slots (2 per k1 for real modes) * P2 <= kMaxWG
 int k2_r2(const Geometry& g, int eb, bool real) {
-  int P2 = 1 << g.l2, P1 = 1 << g.l1;
+  int P2 = 1 << g.l2, P1 = g.p1;
   int r = 8;
   while (r > 1 && ((real ? 2 : 1) * r * P2 > kMaxWG || k2_bytes(r, P2, eb, real) > (size_t)kMaxLds)) r >>= 1;
   return std::min(r, real ? P1 / 2 + 1 : P1);
@@ -1016,9 +1274,12 @@ struct mk_filt_plan {
   Seg* d_hot13 = nullptr;   // hot segments grouped by column length, K1/K3 numbering per group
   int nhot = 0, ncold = 0;
   int64_t n2hot = 0, n2cold = 0, n1cold = 0;
-  int g_off[kHotMaxL1 + 1] = {0}, g_cnt[kHotMaxL1 + 1] = {0};
-  int64_t g_n1[kHotMaxL1 + 1] = {0};
-  void* tw_tab = nullptr;   // stage tables of 2^1..2^11 points (2^l - 2 offsets), then per-P fine/coarse tables
+  int g_off[kNumHotP1] = {0}, g_cnt[kNumHotP1] = {0};  // per column length (kHotP1 index)
+  int64_t g_n1[kNumHotP1] = {0};
+  int64_t g_stab[kNumHotP1] = {0};  // offset of that length's stage table in tw_tab
+  // stage tables of 2^1..2^11 points (2^l - 2 offsets), then per-P fine/coarse
+  // tables, then the mixed-radix stage tables of the hot column lengths
+  void* tw_tab = nullptr;
 };
 
 namespace {
@@ -1035,8 +1296,9 @@ int ensure_tbuf(const mk_filt_plan* cp) {
   return MK_OK;
 }
 
+bool hot_shape(const Seg& s) { return s.l2 == kHotL2 && hot_p1_index(s.p1) >= 0; }
 bool is_hot(const mk_filt_plan* p, const Seg& s) {
-  return p->elem_bytes == 4 && s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1 && s.tw_ptr != nullptr;
+  return p->elem_bytes == 4 && hot_shape(s) && s.tw_ptr != nullptr;
 }
 
 int upload(mk_filt_plan* p) {
@@ -1050,11 +1312,15 @@ int upload(mk_filt_plan* p) {
   for (const Seg& s0 : p->h_segs) {
     Seg s = s0;
     if (is_hot(p, s)) {
-      s.per2 = 1 << (s.l1 - 1);
+      s.per2 = (s.p1 + 1) / 2;  // pairs (k1, P1-k1), k1 = 1..; column 0 (+ P1/2) in item 0
       s.base2 = b_hot;
       b_hot += (int64_t)s.rows * s.per2;
       hot.push_back(s);
     } else {
+      if (s.l1 < 0) {
+        mk::set_error("filter plan: column length %d is only supported on the FP32 hot path", s.p1);
+        return MK_ERR_ARG;
+      }
       s.base2 = b_cold2;
       b_cold2 += (int64_t)s.rows * s.per2;
       s.base1 = s.base3 = b_cold1;
@@ -1062,11 +1328,11 @@ int upload(mk_filt_plan* p) {
       cold.push_back(s);
     }
   }
-  for (int l = 0; l <= kHotMaxL1; l++) {
+  for (int l = 0; l < kNumHotP1; l++) {
     p->g_off[l] = (int)hot13.size();
     int64_t b = 0;
     for (const Seg& s0 : hot)
-      if (s0.l1 == l) {
+      if (s0.p1 == kHotP1[l]) {
         Seg s = s0;
         s.base1 = s.base3 = b;
         b += (int64_t)s.rows * s.per1;
@@ -1097,10 +1363,12 @@ const cx<float>* stage_tab(const mk_filt_plan* p, int l) { return (const cx<floa
 // segment at its P's tables.
 int build_hot_tables(mk_filt_plan* p) {
   std::vector<int64_t> Ps;
+  std::vector<int> p1s;
   for (const Seg& s : p->h_segs)
-    if (s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1) {
-      int64_t P = int64_t(1) << (s.l1 + s.l2);
+    if (hot_shape(s)) {
+      const int64_t P = (int64_t)s.p1 * kHotP2;
       if (std::find(Ps.begin(), Ps.end(), P) == Ps.end()) Ps.push_back(P);
+      if (std::find(p1s.begin(), p1s.end(), s.p1) == p1s.end()) p1s.push_back(s.p1);
     }
   if (Ps.empty()) return MK_OK;
   std::vector<int64_t> offs;
@@ -1108,6 +1376,10 @@ int build_hot_tables(mk_filt_plan* p) {
   for (int64_t P : Ps) {
     offs.push_back(tot);
     tot += (1 << kFineBits) + 2 * P / (1 << kFineBits);
+  }
+  for (int v : p1s) {
+    p->g_stab[hot_p1_index(v)] = tot;
+    tot += v;  // v - 1 entries, rounded up
   }
   if (hipMalloc(&p->tw_tab, tot * sizeof(cx<float>)) != hipSuccess) {
     p->tw_tab = nullptr;
@@ -1117,12 +1389,13 @@ int build_hot_tables(mk_filt_plan* p) {
   cx<float>* base = (cx<float>*)p->tw_tab;
   for (int l = 1; l <= kHotL2; l++) hot_tables_kernel<<<8, 256>>>(base + ((1 << l) - 2), 0, l);
   for (size_t i = 0; i < Ps.size(); i++) hot_tables_kernel<<<64, 256>>>(base + offs[i], Ps[i], 0);
+  for (int v : p1s) mixed_stage_table_kernel<<<1, 256>>>(base + p->g_stab[hot_p1_index(v)], v);
   int r = mk::check_launch("hot_tables_kernel");
   if (r) return r;
   MK_HIP(hipDeviceSynchronize());
   for (Seg& s : p->h_segs)
-    if (s.l2 == kHotL2 && s.l1 >= 1 && s.l1 <= kHotMaxL1) {
-      int64_t P = int64_t(1) << (s.l1 + s.l2);
+    if (hot_shape(s)) {
+      const int64_t P = (int64_t)s.p1 * kHotP2;
       size_t i = std::find(Ps.begin(), Ps.end(), P) - Ps.begin();
       s.tw_ptr = base + offs[i];
     }
@@ -1142,8 +1415,10 @@ void free_filt(mk_filt_plan* p) {
 }
 
 // desc per segment: {rows, n, in_off, in_stride, out_off, out_stride}
+// mixed: FP32 filter plans whose rows may take the hot path's 5-smooth
+// column lengths.
 int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<Seg>& segs, int64_t* n1,
-               int64_t* n2, int64_t* t_elems, int64_t* s_elems, size_t* lds1, size_t* lds2) {
+               int64_t* n2, int64_t* t_elems, int64_t* s_elems, size_t* lds1, size_t* lds2, bool mixed = false) {
   segs.clear();
   int64_t b1 = 0, b2 = 0, toff = 0, soff = 0;
   size_t l1max = 0, l2max = 0;
@@ -1154,16 +1429,16 @@ int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<S
                "segment %d: need rows >= 0 and 2 <= n < 2^30 (rows=%lld n=%lld)", i, (long long)rows,
                (long long)n);
     int64_t m = n - 1;
-    Geometry g = geometry(filter ? (3 * m + 2) / 2 : 2 * m + 1, eb);
-    MK_REQUIRE(g.l1 + g.l2 <= 24, MK_ERR_ARG, "segment %d: n=%lld too long", i, (long long)n);
+    Geometry g = geometry(filter ? (3 * m + 2) / 2 : 2 * m + 1, eb, mixed);
+    MK_REQUIRE(g.P <= (int64_t(1) << 24), MK_ERR_ARG, "segment %d: n=%lld too long", i, (long long)n);
     Seg s{};
     s.in_off = d[2]; s.in_stride = d[3]; s.out_off = d[4]; s.out_stride = d[5];
-    s.rows = (int32_t)rows; s.n = (int32_t)n; s.l1 = g.l1; s.l2 = g.l2; s.cw = g.cw; s.flags = 0;
+    s.rows = (int32_t)rows; s.n = (int32_t)n; s.l1 = g.l1; s.l2 = g.l2; s.cw = g.cw; s.flags = 0; s.p1 = g.p1;
     s.t_off = toff; s.t_stride = g.P;
     s.s_off = soff; s.s_stride = filter ? g.P + 1 : 0;
     s.per1 = (int32_t)((int64_t(1) << g.l2) / g.cw);
     s.r2 = k2_r2(g, 8, filter);  // same grouping for FP32 and FP64 passes
-    s.per2 = (int32_t)mk::cdiv(filter ? (1 << (g.l1 - 1)) + 1 : (1 << g.l1), s.r2);
+    s.per2 = (int32_t)mk::cdiv(filter ? g.p1 / 2 + 1 : g.p1, s.r2);
     s.base1 = b1; s.base2 = b2; s.base3 = b1;
     b1 += rows * s.per1;
     b2 += rows * s.per2;
@@ -1189,32 +1464,30 @@ void launch_fspec(mk_filt_plan* p, const void* wdct, void* tb, size_t lds1, size
   k2_kernel<T, TS, kFSpec><<<(unsigned)p->n2, kThreads, lds2, st>>>(p->d_segs, p->nseg, (cx<T>*)tb);
 }
 
-template <int L1>
+template <int I>
 void launch_hot13(const mk_filt_plan* p, int which, const float* in, float* out, hipStream_t st) {
-  const Seg* segs = p->d_hot13 + p->g_off[L1];
-  const unsigned grid = (unsigned)p->g_n1[L1];
+  constexpr int P1 = kHotP1[I];
+  const Seg* segs = p->d_hot13 + p->g_off[I];
+  const unsigned grid = (unsigned)p->g_n1[I];
+  const cx<float>* stab = (const cx<float>*)p->tw_tab + p->g_stab[I];
   if (which == 1)
-    k1_hot_kernel<L1><<<grid, kThreads, hot13_lds(L1, 1), st>>>(segs, p->g_cnt[L1], in, (cx<float>*)p->tbuf,
-                                                             stage_tab(p, L1));
+    k1_hot_kernel<P1><<<grid, kThreads, hot13_lds(P1, 1), st>>>(segs, p->g_cnt[I], in, (cx<float>*)p->tbuf, stab);
   else
-    k3_hot_kernel<L1><<<grid, kThreads, hot13_lds(L1, 3), st>>>(segs, p->g_cnt[L1], (const cx<float>*)p->tbuf, out,
-                                                             stage_tab(p, L1));
+    k3_hot_kernel<P1><<<grid, kThreads, hot13_lds(P1, 3), st>>>(segs, p->g_cnt[I], (const cx<float>*)p->tbuf, out,
+                                                             stab);
+}
+
+template <int I = 0>
+void launch_hot13_idx(const mk_filt_plan* p, int idx, int which, const float* in, float* out, hipStream_t st) {
+  if constexpr (I < kNumHotP1) {
+    if (idx == I) launch_hot13<I>(p, which, in, out, st);
+    else launch_hot13_idx<I + 1>(p, idx, which, in, out, st);
+  }
 }
 
 void launch_hot13_all(const mk_filt_plan* p, int which, const float* in, float* out, hipStream_t st) {
-  for (int l = 1; l <= kHotMaxL1; l++) {
-    if (!p->g_cnt[l] || !p->g_n1[l]) continue;
-    switch (l) {
-      case 1: launch_hot13<1>(p, which, in, out, st); break;
-      case 2: launch_hot13<2>(p, which, in, out, st); break;
-      case 3: launch_hot13<3>(p, which, in, out, st); break;
-      case 4: launch_hot13<4>(p, which, in, out, st); break;
-      case 5: launch_hot13<5>(p, which, in, out, st); break;
-      case 6: launch_hot13<6>(p, which, in, out, st); break;
-      case 7: launch_hot13<7>(p, which, in, out, st); break;
-      default: launch_hot13<8>(p, which, in, out, st); break;
-    }
-  }
+  for (int i = 0; i < kNumHotP1; i++)
+    if (p->g_cnt[i] && p->g_n1[i]) launch_hot13_idx(p, i, which, in, out, st);
 }
 
 template <typename T, typename TIO>
@@ -1256,8 +1529,9 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
   p->nseg = nseg;
   p->borrowed.assign(std::max(1, nseg), 0);
   p->elem_bytes = elem_bytes;
+  const bool hot = elem_bytes == 4 && getenv("MK_FFT_NO_HOT") == nullptr;
   int st = build_segs(nseg, desc, elem_bytes, true, p->h_segs, &p->n1, &p->n2, &p->t_elems, &p->s_elems, &p->lds1,
-                      &p->lds2);
+                      &p->lds2, hot);
   if (st) { free_filt(p); return st; }
   MK_REQUIRE(p->n1 < INT32_MAX && p->n2 < INT32_MAX, (free_filt(p), MK_ERR_ARG), "too many work items");
   if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
@@ -1270,7 +1544,7 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
     return MK_ERR_ALLOC;
   }
   for (auto& sg : p->h_segs) sg.s_ptr = (unsigned char*)p->spec + sg.s_off * elem_bytes;
-  if (elem_bytes == 4 && getenv("MK_FFT_NO_HOT") == nullptr) {
+  if (hot) {
     st = build_hot_tables(p);
     if (st) { free_filt(p); return st; }
   }
@@ -1289,7 +1563,7 @@ extern "C" int mk_filt_plan_borrow_spectrum(mk_filt_plan* p, int seg, const mk_f
   MK_REQUIRE(src->have_spec, MK_ERR_ARG, "source plan has no weights");
   Seg& d = p->h_segs[seg];
   const Seg& s = src->h_segs[src_seg];
-  MK_REQUIRE(d.rows == s.rows && d.n == s.n && d.l1 == s.l1 && d.l2 == s.l2, MK_ERR_ARG,
+  MK_REQUIRE(d.rows == s.rows && d.n == s.n && d.p1 == s.p1 && d.l2 == s.l2, MK_ERR_ARG,
              "segment geometry differs (rows %d/%d, n %d/%d)", d.rows, s.rows, d.n, s.n);
   d.s_ptr = s.s_ptr;
   p->borrowed[seg] = 1;
@@ -1322,7 +1596,7 @@ extern "C" int mk_filt_plan_geometry(const mk_filt_plan* p, int64_t* out) {
   MK_REQUIRE(p && out, MK_ERR_ARG, "null pointer");
   for (int i = 0; i < p->nseg; i++) {
     const Seg& s = p->h_segs[i];
-    out[4 * i] = int64_t(1) << s.l1;
+    out[4 * i] = s.p1;
     out[4 * i + 1] = int64_t(1) << s.l2;
     out[4 * i + 2] = s.cw;
     out[4 * i + 3] = s.t_stride;
@@ -1340,7 +1614,7 @@ extern "C" int mk_filt_plan_set_weights(mk_filt_plan* p, const double* wdct, voi
   // and stored in the plan's precision, unless FP64 LDS tiles would not fit.
   size_t l1_64 = 0, l2_64 = 0;
   for (const Seg& sg : p->h_segs) {
-    Geometry g{sg.l1, sg.l2, sg.cw, int64_t(1) << (sg.l1 + sg.l2)};
+    Geometry g{sg.l1, sg.l2, sg.cw, sg.p1, (int64_t)sg.p1 << sg.l2};
     l1_64 = std::max(l1_64, lds_k13(g, 8));
     l2_64 = std::max(l2_64, lds_k2(g, 8));
   }
