@@ -820,32 +820,42 @@ __global__ void __launch_bounds__(kThreads, 4) k2_hot_kernel(const Seg* __restri
 
 // ---- K1 / K3, FP32 filter hot path: columns of P1 = 2^a 3^b 5^c (rows of 2^11)
 // One launch per column length; CW adjacent columns per workgroup (the
-// largest power of two with CW P1 <= 4096), so global reads and writes are
-// runs of CW points.  The column FFT is a compile-time mixed-radix Stockham
-// sequence (next_radix order) over the CW padded rows.
+// largest power of two with CW P1 <= 4096, but at least 32), so global reads
+// and writes are runs of CW points.  The LDS tile is column-major: point j
+// of column c at j CW + c.  A stage's butterflies are numbered column-fastest,
+// so the lanes of a half-wave touch 32 consecutive columns of one row (no
+// bank conflicts, one shared twiddle), and every index is a shift or mask of
+// the butterfly number.
+constexpr int kHotMaxPts = 8192;  // points per K1/K3 workgroup (16 per thread)
 constexpr int hot_cw(int p1) {
+  int c = 1;
+  while (2 * c * p1 <= kMaxWG && 2 * c <= kHotP2) c *= 2;
+  return c < 32 ? 32 : c;
+}
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+// Column tile of the general kernels (the spectrum pass runs on the plan's
+// geometry): the largest power of two with cw P1 <= 4096.
+constexpr int generic_cw(int p1) {
   int c = 1;
   while (2 * c * p1 <= kMaxWG && 2 * c <= kHotP2) c *= 2;
   return c;
 }
-constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
-// One radix-R stage of an N-point FFT over NROWS padded rows (stride RS);
-// the tail of the last butterfly round is masked when NROWS N / R is not a
-// multiple of the workgroup size.
-template <bool INV, int R, int N, int NS, int NROWS, int RS>
-__device__ __forceinline__ void ct_stage_mx(cx<float>* buf, const cx<float>* tw) {
-  constexpr int Q = N / R, TOT = NROWS * Q, NB = (TOT + kThreads - 1) / kThreads;
+// One radix-R stage of N-point column FFTs over the CW columns of a
+// column-major tile; the tail of the last butterfly round is masked when
+// CW N / R is not a multiple of the workgroup size.
+template <bool INV, int R, int N, int NS, int CW>
+__device__ __forceinline__ void ct_stage_cm(cx<float>* buf, const cx<float>* tw) {
+  constexpr int Q = N / R, TOT = CW * Q, NB = (TOT + kThreads - 1) / kThreads, LCW = ilog2c(CW);
   constexpr bool FULL = NB * kThreads == TOT;
   cx<float> v[NB][R];
 #pragma unroll
   for (int u = 0; u < NB; u++) {
     const int bf = threadIdx.x + u * kThreads;
     if (FULL || bf < TOT) {
-      const int row = bf / Q, j = bf - row * Q, k = j % NS;
-      const cx<float>* base = buf + row * RS;
+      const int c = bf & (CW - 1), j = bf >> LCW, k = j % NS;
 #pragma unroll
-      for (int r = 0; r < R; r++) v[u][r] = base[pad8(j + r * Q)];
+      for (int r = 0; r < R; r++) v[u][r] = buf[(j + r * Q) * CW + c];
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
@@ -861,41 +871,41 @@ __device__ __forceinline__ void ct_stage_mx(cx<float>* buf, const cx<float>* tw)
   for (int u = 0; u < NB; u++) {
     const int bf = threadIdx.x + u * kThreads;
     if (FULL || bf < TOT) {
-      const int row = bf / Q, j = bf - row * Q, k = j % NS;
+      const int c = bf & (CW - 1), j = bf >> LCW, k = j % NS;
       dft_r<float, INV, R>(v[u]);
-      cx<float>* base = buf + row * RS;
       const int idx = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) base[pad8(idx + r * NS)] = v[u][r];
+      for (int r = 0; r < R; r++) buf[(idx + r * NS) * CW + c] = v[u][r];
     }
   }
   __syncthreads();
 }
 
-template <bool INV, int N, int NROWS, int RS, int NS = 1, int OFF = 0>
-__device__ __forceinline__ void ct_fft_mx(cx<float>* buf, const cx<float>* st) {
+template <bool INV, int N, int CW, int NS = 1, int OFF = 0>
+__device__ __forceinline__ void ct_fft_cm(cx<float>* buf, const cx<float>* st) {
   if constexpr (NS < N) {
     constexpr int R = next_radix(N / NS);
-    ct_stage_mx<INV, R, N, NS, NROWS, RS>(buf, st + OFF);
-    ct_fft_mx<INV, N, NROWS, RS, NS * R, OFF + (R - 1) * NS>(buf, st);
+    ct_stage_cm<INV, R, N, NS, CW>(buf, st + OFF);
+    ct_fft_cm<INV, N, CW, NS * R, OFF + (R - 1) * NS>(buf, st);
   }
 }
 
 template <int P1>
-__global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(kThreads, 2) k1_hot_kernel(const Seg* __restrict__ segs, int nseg,
                                                               const float* __restrict__ in,
                                                               cx<float>* __restrict__ tbuf,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), RS = row_stride(P1), NPTS = CW * P1;
+  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), NPTS = CW * P1;
   constexpr int P = P1 * kHotP2;
+  static_assert(NPTS <= kHotMaxPts && kHotP2 % CW == 0, "K1 tile");
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 1);
   const Seg sg = segs[s];
   const int local = (int)((int64_t)blockIdx.x - sg.base1);
   const int row = local / sg.per1, jb = local - row * sg.per1;
   const int m = sg.n - 1;
-  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
-  cx<float>* st = buf + CW * RS;                             // [P1] stage table
+  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [P1][CW]
+  cx<float>* st = buf + NPTS;                                // [P1] stage table
   cx<float>* F = st + P1;                                    // [1024] w_2P^l
   cx<float>* Ct = F + (1 << kFineBits);                      // [4 P1] w_2P^(1024 h)
   const cx<float>* twF = reinterpret_cast<const cx<float>*>(sg.tw_ptr);
@@ -904,7 +914,8 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
   for (int e = threadIdx.x; e < 4 * P1; e += kThreads) Ct[e] = twF[(1 << kFineBits) + e];
   const float* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * CW;
-  // real input a_i, i = 2 (P2 j1 + j2) + w, from the even extension on [-m, 2m]
+  // real input a_i, i = 2 (P2 j1 + j2) + w, from the even extension on
+  // [-m, 2m]; reals 2 (j1 CW + c) + {0, 1} of the tile are consecutive
   constexpr int NR = 2 * NPTS, NV = (NR + kThreads - 1) / kThreads;
   float val[NV];
 #pragma unroll
@@ -921,11 +932,10 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
 #pragma unroll
   for (int u = 0; u < NV; u++) {
     const int idx = threadIdx.x + u * kThreads;
-    const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
-    if (NV * kThreads == NR || idx < NR) reinterpret_cast<float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] = val[u];
+    if (NV * kThreads == NR || idx < NR) reinterpret_cast<float*>(buf)[idx] = val[u];
   }
   __syncthreads();
-  ct_fft_mx<false, P1, CW, RS>(buf, st);
+  ct_fft_cm<false, P1, CW>(buf, st);
   cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   constexpr int NO = (NPTS + kThreads - 1) / kThreads;
 #pragma unroll
@@ -934,25 +944,26 @@ __global__ void __launch_bounds__(kThreads, 4) k1_hot_kernel(const Seg* __restri
     if (NO * kThreads != NPTS && idx >= NPTS) break;
     const int k1 = idx >> LCW, c = idx & (CW - 1);
     const int a = ((j2_0 + c) * k1) % P;  // < 2^11 * 2^8
-    trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[c * RS + pad8(k1)], tw2P(F, Ct, 2 * a));
+    trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[idx], tw2P(F, Ct, 2 * a));
   }
 }
 
 template <int P1>
-__global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(kThreads, 2) k3_hot_kernel(const Seg* __restrict__ segs, int nseg,
                                                               const cx<float>* __restrict__ tbuf,
                                                               float* __restrict__ out,
                                                               const cx<float>* __restrict__ stab_g) {
-  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), RS = row_stride(P1), NPTS = CW * P1;
+  constexpr int CW = hot_cw(P1), LCW = ilog2c(CW), NPTS = CW * P1;
   constexpr int P = P1 * kHotP2;
+  static_assert(NPTS <= kHotMaxPts && kHotP2 % CW == 0, "K3 tile");
   extern __shared__ unsigned char smem_raw[];
   const int s = find_seg(segs, nseg, blockIdx.x, 3);
   const Seg sg = segs[s];
   const int local = (int)((int64_t)blockIdx.x - sg.base3);
   const int row = local / sg.per1, jb = local - row * sg.per1;
   const int m = sg.n - 1;
-  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [CW][RS]
-  cx<float>* st = buf + CW * RS;                             // [P1] stage table
+  cx<float>* buf = reinterpret_cast<cx<float>*>(smem_raw);  // [P1][CW]
+  cx<float>* st = buf + NPTS;                                // [P1] stage table
   for (int e = threadIdx.x; e < P1 - 1; e += kThreads) st[e] = stab_g[e];
   const cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   const int j2_0 = jb * CW;
@@ -967,11 +978,11 @@ __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restri
 #pragma unroll
     for (int u = 0; u < NV; u++) {
       const int idx = threadIdx.x + u * kThreads;
-      if (NV * kThreads == NPTS || idx < NPTS) buf[(idx & (CW - 1)) * RS + pad8(idx >> LCW)] = v[u];
+      if (NV * kThreads == NPTS || idx < NPTS) buf[idx] = v[u];
     }
   }
   __syncthreads();
-  ct_fft_mx<true, P1, CW, RS>(buf, st);
+  ct_fft_cm<true, P1, CW>(buf, st);
   const float scale = 1.f / (float)P;
   const bool halve = sg.flags & 1;
   float* y = out + sg.out_off + (int64_t)row * sg.out_stride;
@@ -983,7 +994,7 @@ __global__ void __launch_bounds__(kThreads, 4) k3_hot_kernel(const Seg* __restri
     const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
     const int k = 2 * (kHotP2 * j1 + j2_0) + w - m;
     if (k >= 0 && k <= m) {
-      float v = reinterpret_cast<const float*>(buf + (w >> 1) * RS + pad8(j1))[w & 1] * scale;
+      float v = reinterpret_cast<const float*>(buf)[idx] * scale;
       if (halve && (k == 0 || k == m)) v *= 0.5f;
       y[k] = v;
     }
@@ -1004,8 +1015,7 @@ int hot_p1_index(int p1) {
 // LDS of the hot K1 (tile, stage table, fine/coarse tables) and K3 (tile,
 // stage table).
 size_t hot13_lds(int p1, int which) {
-  const int CW = hot_cw(p1), RS = row_stride(p1);
-  return (size_t)(CW * RS + p1 + (which == 1 ? (1 << kFineBits) + 4 * p1 : 0)) * sizeof(cx<float>);
+  return (size_t)(hot_cw(p1) * p1 + p1 + (which == 1 ? (1 << kFineBits) + 4 * p1 : 0)) * sizeof(cx<float>);
 }
 
 // Stage table of an N-point mixed-radix FFT (fill_stage_table_mx order),
@@ -1161,7 +1171,7 @@ Geometry geometry(int64_t P_min, int elem_bytes, bool mixed = false) {
         g.l1 = (g.p1 & (g.p1 - 1)) == 0 ? ilog2c(g.p1) : -1;
         g.l2 = kHotL2;
         g.P = (int64_t)g.p1 * kHotP2;
-        g.cw = hot_cw(g.p1);
+        g.cw = generic_cw(g.p1);
         return g;
       }
   }
@@ -1334,6 +1344,7 @@ int upload(mk_filt_plan* p) {
     for (const Seg& s0 : hot)
       if (s0.p1 == kHotP1[l]) {
         Seg s = s0;
+        s.per1 = kHotP2 / hot_cw(s.p1);  // the hot K1/K3 tile is wider than the general one
         s.base1 = s.base3 = b;
         b += (int64_t)s.rows * s.per1;
         hot13.push_back(s);

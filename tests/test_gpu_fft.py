@@ -60,8 +60,9 @@ def test_dct1_involution_large(env):
     assert rel_err(y / (2 * (n - 1)), x) < 1e-12
 
 
-def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False):
-    """Batched filter over segments of (rows, n); returns gpu, ref."""
+def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False, geometry=None):
+    """Batched filter over segments of (rows, n); returns gpu, ref, info.
+    geometry: optional list that receives each segment's (P1, P2, cw, P)."""
     torch, device, mkgpu = env
     rng = np.random.default_rng(seed)
     desc = []
@@ -91,6 +92,10 @@ def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False):
         out = yt.cpu().numpy()
         info = np.zeros(8, dtype=np.int64)
         mkgpu.check("mk_filt_plan_info", h, info.ctypes.data)
+        if geometry is not None:
+            g = np.zeros(4 * len(rows_n), dtype=np.int64)
+            mkgpu.check("mk_filt_plan_geometry", h, g.ctypes.data)
+            geometry.extend(tuple(int(v) for v in r) for r in g.reshape(-1, 4))
     finally:
         mkgpu.check("mk_filt_plan_destroy", h)
     refs = []
@@ -166,3 +171,36 @@ def test_filter_fp32_hot_and_general_paths_agree(env):
     assert rel_err(out_hot, ref) < 5e-6
     assert rel_err(out_gen, ref) < 5e-6
     assert rel_err(out_hot, out_gen) < 5e-6
+
+
+def _smooth(v):
+    for d in (2, 3, 5):
+        while v % d == 0:
+            v //= d
+    return v == 1
+
+
+def test_filter_fp32_every_hot_column_length(env):
+    """One row per hot column length P1 (2^a 3^b 5^c <= 256): n is chosen so
+    that the smallest 5-smooth P1 with 2^11 P1 >= (3(n-1)+2)/2 is exactly that
+    value, the plan's reported geometry confirms it, and each row matches the
+    FP64 DCT-I pair and the power-of-two general path (MK_FFT_NO_HOT)."""
+    import os
+    p1s = [v for v in range(2, 257) if _smooth(v)]
+    assert len(p1s) == 51
+    segs = [(1, (2 * 2048 * p1 - 2) // 3 + 1) for p1 in p1s]
+    geo = []
+    out, ref, _ = _filter(env, segs, np.float32, halve=True, seed=11, geometry=geo)
+    assert [g[0] for g in geo] == p1s
+    assert all(g[1] == 2048 and g[3] == g[0] * 2048 for g in geo)
+    os.environ["MK_FFT_NO_HOT"] = "1"
+    try:
+        out_gen, _, _ = _filter(env, segs, np.float32, halve=True, seed=11)
+    finally:
+        del os.environ["MK_FFT_NO_HOT"]
+    off = 0
+    for (rows, n), p1 in zip(segs, p1s):
+        o, r, g = out[off:off + n], ref[off:off + n], out_gen[off:off + n]
+        assert rel_err(o, r) < 5e-6, (p1, n, rel_err(o, r))
+        assert rel_err(o, g) < 5e-6, (p1, n)
+        off += n
