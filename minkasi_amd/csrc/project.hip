@@ -24,21 +24,54 @@ namespace {
 // ---------------------------------------------------------------------------
 // map2tod
 
+// 4 samples per thread per step (one int4 of ipix, one 4-wide store of dat),
+// U steps per loop trip with all their index loads, then all their gathers,
+// in flight together; the tail is handled scalar.
+template <typename T>
+struct alignas(4 * sizeof(T)) Quad { T v[4]; };
+
 template <typename T>
 __global__ void __launch_bounds__(256) map2tod_kernel(T* __restrict__ dat, const T* __restrict__ map, int64_t n,
                                                       const int32_t* __restrict__ ipix, int do_add) {
-  // 4 samples per thread per step; the tail is handled scalar.
-  int64_t n4 = n >> 2;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  constexpr int U = 8;
+  const int64_t n4 = n >> 2;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int4* ip4 = reinterpret_cast<const int4*>(ipix);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
-    int4 p = ip4[q];
-    T v0 = map[p.x], v1 = map[p.y], v2 = map[p.z], v3 = map[p.w];
-    T* d = dat + 4 * q;
-    if (do_add) {
-      v0 += d[0]; v1 += d[1]; v2 += d[2]; v3 += d[3];
+  Quad<T>* d4 = reinterpret_cast<Quad<T>*>(dat);
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q + (U - 1) * stride < n4; q += U * stride) {
+    int4 p[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) p[u] = ip4[q + u * stride];
+    Quad<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      v[u].v[0] = map[p[u].x];
+      v[u].v[1] = map[p[u].y];
+      v[u].v[2] = map[p[u].z];
+      v[u].v[3] = map[p[u].w];
     }
-    d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3;
+    if (do_add) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const Quad<T> o = d4[q + u * stride];
+#pragma unroll
+        for (int e = 0; e < 4; e++) v[u].v[e] += o.v[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) d4[q + u * stride] = v[u];
+  }
+  for (; q < n4; q += stride) {
+    const int4 p = ip4[q];
+    Quad<T> v;
+    v.v[0] = map[p.x]; v.v[1] = map[p.y]; v.v[2] = map[p.z]; v.v[3] = map[p.w];
+    if (do_add) {
+      const Quad<T> o = d4[q];
+#pragma unroll
+      for (int e = 0; e < 4; e++) v.v[e] += o.v[e];
+    }
+    d4[q] = v;
   }
   for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     T v = map[ipix[i]];
@@ -61,7 +94,7 @@ int map2tod_impl(T* dat, const T* map, int64_t nsamp, const int32_t* ipix, int d
   MK_REQUIRE(dat && map && ipix, MK_ERR_ARG, "null pointer");
   MK_REQUIRE(((uintptr_t)ipix & 15) == 0 && ((uintptr_t)dat % (4 * sizeof(T))) == 0, MK_ERR_ARG,
              "map2tod: ipix must be 16-byte aligned and dat aligned to 4 elements");
-  map2tod_kernel<T><<<mk::grid_for(mk::cdiv(nsamp, 4), 256, 256 * 16), 256, 0, mk::as_stream(stream)>>>(
+  map2tod_kernel<T><<<mk::grid_for(mk::cdiv(nsamp, 32), 256, 8192), 256, 0, mk::as_stream(stream)>>>(
       dat, map, nsamp, ipix, do_add);
   return mk::check_launch("map2tod_kernel");
 }
@@ -250,7 +283,7 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
 // ordered by local pixel, so every pixel is one run; the run's last lane
 // adds the run sum into the wave's private LDS tile (one add per pixel per
 // group, in group order: the result does not depend on timing).
-constexpr int kT2MUnroll = 4;
+constexpr int kT2MUnroll = 16;
 
 template <typename T, typename P>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
