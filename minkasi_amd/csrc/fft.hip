@@ -1,4 +1,4 @@
-// Batched power-of-two four-step FFT engine and the two transforms built on it.
+// Batched four-step FFT engine and the two transforms built on it.
 //
 // NoiseSmoothedSVD.apply_noise (minkasi/mapmaking/noise.py:704-711) applies,
 // per rotated detector mode y (length n), y'' = C W C y where C is the
@@ -7,11 +7,13 @@
 // m = n-1 and N = 2m, C W C y = (even extension of y) circularly convolved
 // (period N) with h = C W (because C C = N I).  The engine evaluates that
 // circular convolution exactly as a linear one through a real FFT of
-// L = 2^q >= 3m+1 points: the input is the even periodic extension z'
-// on [-m, 2m], the kernel is h placed symmetrically (so its spectrum B is
-// real and even), and the outputs needed are the window [m, 2m].  This works
-// for any n (no Bluestein, no smooth-length requirement) and fuses the
-// forward transform, the weight multiply and the inverse transform.
+// L >= 3m+1 points: the input is the even periodic extension z' on
+// [-m, 2m], the kernel is h placed symmetrically (so its spectrum B is real
+// and even), and the outputs needed are the window [m, 2m].  This works for
+// any n (no Bluestein, no smooth-length requirement) and fuses the forward
+// transform, the weight multiply and the inverse transform.  L is the
+// smallest 4096 P1 >= 3m+1 with P1 5-smooth (<= 256) on the FP32 hot path,
+// else the next power of two.
 //
 // Stand-alone DCT-I (fft_r2r, minkasi/tools/fft.py:290-344) uses Bluestein's
 // chirp-z algorithm (Bluestein 1970; Rabiner, Schafer & Rader 1969) on the
@@ -26,9 +28,10 @@
 //       real-FFT unpacking partner C_(P-k) lives), pointwise spectral work,
 //       inverse row FFTs, conjugate twiddle, store back.
 //   K3: inverse column FFTs of length P1, 1/P scale, write the wanted outputs.
-// Every LDS sub-FFT is a radix-4/radix-2 Stockham pass sequence with a
-// per-workgroup twiddle table.  Each launch covers every row of every
-// segment (TOD) in the batch through a work list.
+// Every LDS sub-FFT is a Stockham pass sequence (radix 8, then 4 or 2; radix
+// 3 and 5 for mixed column lengths) with a stage-ordered twiddle table.
+// Each launch covers every row of every segment (TOD) in the batch through
+// a work list.
 #include "common.h"
 
 #include <algorithm>
