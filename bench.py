@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--truncate", action="store_true",
+                    help="cut every TOD so that n-1 is {2,3,5,7,11}-smooth first (truncate_tod)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3: setup + steps only")
     ap.add_argument("--weights", default=os.path.join(HERE, "minkasi_amd", "learned", "weights_c4.npz"),
                     help="trained learned-preconditioner weights")
@@ -154,7 +156,7 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t_setup = time.time()
-    P = build_problem(args.config, ntod=args.ntod, verbose=True)
+    P = build_problem(args.config, ntod=args.ntod, verbose=True, truncate=args.truncate)
     log("setup: generate %.1fs pixelize %.1fs noise %.1fs plans %.1fs rhs %.1fs; %d TODs, %.3g samples" % (
         P.t_generate, P.t_pixelize, P.t_noise, P.t_plans, P.t_rhs, len(P.shapes), P.nsamp))
     op = P.op
@@ -199,9 +201,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "config%d: %d TODs, ndet %s, n %s, %dx%d map at %g arcsec, NoiseSmoothedSVD, "
-                               "Jacobi PCG" % (args.config, len(P.shapes), CONFIGS[args.config]["ndet"],
-                                               CONFIGS[args.config]["n"], P.map.nx, P.map.ny,
-                                               CONFIGS[args.config]["pix"]),
+                               "Jacobi PCG%s" % (args.config, len(P.shapes), CONFIGS[args.config]["ndet"],
+                                                 CONFIGS[args.config]["n"], P.map.nx, P.map.ny,
+                                                 CONFIGS[args.config]["pix"],
+                                                 ", truncate_tod lengths" if args.truncate else ""),
                    "ntod": len(P.shapes), "nsamp": int(P.nsamp), "map": [P.map.nx, P.map.ny]},
         "msamp_per_s": round(P.nsamp * iters_per_s / 1e6, 1),
         "setup_s": round(t_setup, 1),

@@ -15,7 +15,9 @@ class Problem:
     pass
 
 
-def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False):
+def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False, truncate=False):
+    """truncate: cut every TOD so that n - 1 is {2,3,5,7,11}-smooth, as the
+    reference's examples do before mapmaking (tods/processing.py:648-681)."""
     from .maps.mapset import Mapset
     from .maps.skymap import SkyMap
     from .mapmaking.noise import NoiseSmoothedSVD
@@ -33,7 +35,13 @@ def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False)
     for i, (ndet, n) in enumerate(shapes):
         info = make_tod_info(config_id, seed_base + i, ndet, n, spec, dtype=dtype, backend="torch",
                              want_radec=False)
+        if truncate:
+            from .tods.processing import truncate_tod
+            truncate_tod(info)
         todvec.add_tod(Tod(info))
+    if truncate:
+        shapes = [tod.get_data_dims() for tod in todvec.tods]
+        P.shapes = shapes
     torch.cuda.synchronize()
     P.t_generate = time.time() - t0
     t0 = time.time()
