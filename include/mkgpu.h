@@ -329,6 +329,46 @@ int mk_adam_f32(float *p, const float *g, float *m, float *v, int32_t n,
                 const double *dalpha, float lr, float beta1, float beta2,
                 float eps, int32_t step, float gscale, void *stream);
 
+/* ---- TOD preprocessing (examples/minkasi_mpi_example.py:42-52) ----------
+ * fit_cm_plus_poly (minkasi/tods/processing.py:510-579) in device pieces.
+ * Per column c (sample) of a row-major nrow x ncol block (row stride ld):
+ * out[c] = np.median over the rows (the mean of the two middle values for an
+ * even count); 1 <= nrow <= 1024.  Replaces np.median(dd, axis=0)
+ * (processing.py:566). */
+int mk_median_cols_f32(const float *dat, int64_t nrow, int64_t ncol,
+                       int64_t ld, float *out, void *stream);
+int mk_median_cols_f64(const double *dat, int64_t nrow, int64_t ncol,
+                       int64_t ld, double *out, void *stream);
+/* Design of the fit (processing.py:556-570): with x_t = (t - (n-1)/2) /
+ * ((n-1)/2), X[t][k] = P_k(x_t) for k <= ord (legvander(xx, ord)) and
+ * X[t][ord+1+i] = P_i(x_t) cm[t] for i < cm_ord (cm_pmat * cm); X is n x ncol
+ * row-major and Xt its transpose, FP64, ncol = ord + 1 + cm_ord <= 8. */
+int mk_cm_poly_design_f32(const float *cm, int64_t n, int ord, int cm_ord,
+                          double *X, double *Xt, void *stream);
+int mk_cm_poly_design_f64(const double *cm, int64_t n, int ord, int cm_ord,
+                          double *X, double *Xt, void *stream);
+/* D[k][j] = sum_t X[t][k] y[j][t] (ncol x nrow, FP64, fixed-order sums):
+ * the normal-equation pieces mm^T mm and mm^T dat^T of _linfit_2mat
+ * (processing.py:28-39).  work: mk_design_dots_workspace_bytes bytes. */
+int64_t mk_design_dots_workspace_bytes(int64_t n, int ncol, int64_t nrow);
+int mk_design_dots_f32(const double *X, int64_t n, int ncol, const float *y,
+                       int64_t nrow, int64_t ld, double *D, void *work,
+                       int64_t work_bytes, void *stream);
+int mk_design_dots_f64(const double *X, int64_t n, int ncol, const double *y,
+                       int64_t nrow, int64_t ld, double *D, void *work,
+                       int64_t work_bytes, void *stream);
+/* out[j][t] = base[j][t] - sum_{k<nk} X[t][c0+k] coef[k][j], or the sum
+ * alone when base is NULL: dd = dat - pmat.fitp_p and pred2 = cm_mat.fitp_cm
+ * (processing.py:573-575); coef is nk x nrow. */
+int mk_design_apply_f32(const float *base, int64_t nrow, int64_t n, int64_t ld,
+                        const double *X, int ncol, int c0, int nk,
+                        const double *coef, float *out, int64_t ld_out,
+                        void *stream);
+int mk_design_apply_f64(const double *base, int64_t nrow, int64_t n,
+                        int64_t ld, const double *X, int ncol, int c0, int nk,
+                        const double *coef, double *out, int64_t ld_out,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,15 @@ _SIGS = {
     "mk_gram_workspace_bytes": ([I64, I64], I64),
     "mk_gram_f64": ([I64, I64, P, I, I64, P, P, I64, P], I),
     "mk_smooth_rows_f64": ([P, P, P, I64, I64, P, I32, I32, I32, P], I),
+    "mk_median_cols_f32": ([P, I64, I64, I64, P, P], I),
+    "mk_median_cols_f64": ([P, I64, I64, I64, P, P], I),
+    "mk_cm_poly_design_f32": ([P, I64, I32, I32, P, P, P], I),
+    "mk_cm_poly_design_f64": ([P, I64, I32, I32, P, P, P], I),
+    "mk_design_dots_workspace_bytes": ([I64, I32, I64], I64),
+    "mk_design_dots_f32": ([P, I64, I32, P, I64, I64, P, P, I64, P], I),
+    "mk_design_dots_f64": ([P, I64, I32, P, I64, I64, P, P, I64, P], I),
+    "mk_design_apply_f32": ([P, I64, I64, I64, P, I32, I32, I32, P, P, I64, P], I),
+    "mk_design_apply_f64": ([P, I64, I64, I64, P, I32, I32, I32, P, P, I64, P], I),
 }
 
 _fns = {}

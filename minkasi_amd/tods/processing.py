@@ -1,4 +1,6 @@
-"""TOD preprocessing: truncation to an FFT-friendly length.
+"""TOD preprocessing of minkasi's canonical pipeline
+(examples/minkasi_mpi_example.py:42-52): truncation to an FFT-friendly
+length, DCT-I downsampling and the common-mode plus polynomial-drift fit.
 
 truncate_tod follows minkasi/tods/processing.py:648-681.  The TOD keeps
 ndata_new = max(find_good_fft_lens(ndata - 1, primes)) + 1 samples (nothing
@@ -17,6 +19,10 @@ the TOD as it is, like the reference.
 import math
 
 import numpy as np
+import torch
+
+from .. import device as dev
+from ..lib import mkgpu
 
 
 def _products(lim, logs, start, out):
@@ -69,3 +75,88 @@ def truncate_tod(tod_info, primes=(2, 3, 5, 7, 11)):
         else:
             tod_info[key] = np.take(a, indices=range(0, new), axis=axes[0])
     return new
+
+
+def downsample_tod(tod_info, fac=10):
+    """DCT-I downsampling of every array whose last axis has ndata samples
+    (processing.py:621-645, via downsample_array_r2r).  As in the reference,
+    dt is multiplied by 10 whatever fac is (processing.py:637)."""
+    from ..tools.array_ops import downsample_array_r2r
+    ndata = tod_info["dat_calib"].shape[1]
+    tod_info["dt"] *= 10
+    for key in list(tod_info.keys()):
+        a = tod_info[key]
+        shape = getattr(a, "shape", None)
+        if shape is None or len(shape) == 0 or shape[-1] != ndata:
+            continue
+        tod_info[key] = downsample_array_r2r(a, fac)
+
+
+def fit_cm_plus_poly(dat, ord=2, cm_ord=1, niter=1, medsub=False, full_out=False):
+    """Fit the detector-median common mode times Legendre polynomials of order
+    < cm_ord plus per-detector Legendre drifts of order <= ord, and return
+    the data with the drifts (not the common mode) removed
+    (processing.py:510-579).
+
+    Device tensors (ndet x ndata, FP32 or FP64) are fitted on the GPU and
+    keep their dtype; numpy input is fitted in FP64 and returned as FP64
+    numpy arrays, as the reference's float64 arithmetic returns them.  The
+    median (mk_median_cols_*), the design, the normal-equation sums
+    (mk_design_dots_*) and the subtraction (mk_design_apply_*) run on the
+    device; the (ord + 1 + cm_ord)^2 inverse is taken on the host in FP64,
+    as np.linalg.inv does in _linfit_2mat (processing.py:28-39).
+    full_out: also return pred2 (the fitted common mode, ndet x ndata) and cm."""
+    if medsub:
+        raise NotImplementedError("medsub=True (a per-detector median over ndata samples) is not provided")
+    if ord < 0 or cm_ord < 1:
+        raise ValueError("need ord >= 0 and cm_ord >= 1 (legvander of a negative degree)")
+    host = not isinstance(dat, torch.Tensor)
+    if host:
+        d = torch.from_numpy(np.ascontiguousarray(dat, dtype=np.float64)).to(dev.device())
+    else:
+        d = dat.to(dev.device())
+        if d.dtype not in (torch.float32, torch.float64):
+            d = d.to(torch.float64)
+        d = d.contiguous()
+    if d.dim() != 2:
+        raise ValueError("dat must be ndet x ndata")
+    ndet, n = d.shape
+    if n < 2 or ndet < 1 or ndet > 1024:
+        raise ValueError("need ndata >= 2 and 1 <= ndet <= 1024 (got %d x %d)" % (ndet, n))
+    sfx = "f64" if d.dtype == torch.float64 else "f32"
+    ncol = ord + 1 + cm_ord
+    if ncol > 8:
+        raise ValueError("ord + 1 + cm_ord must be at most 8")
+    f64 = dict(dtype=torch.float64, device=d.device)
+    sp = dev.stream_ptr()
+    dd = d.clone()
+    cm = torch.zeros(n, dtype=d.dtype, device=d.device)
+    pred2 = torch.zeros(n, dtype=d.dtype, device=d.device)  # the reference's initial value (niter = 0)
+    X = torch.empty((n, ncol), **f64)
+    Xt = torch.empty((ncol, n), **f64)
+    lhs = torch.empty((ncol, ncol), **f64)
+    rhs = torch.empty((ncol, ndet), **f64)
+    wb = max(int(mkgpu.lib.mk_design_dots_workspace_bytes(n, ncol, ndet)),
+             int(mkgpu.lib.mk_design_dots_workspace_bytes(n, ncol, ncol)))
+    work = torch.empty(wb // 8 + 32, **f64)
+    for _ in range(niter):
+        mkgpu.check("mk_median_cols_" + sfx, dd.data_ptr(), ndet, n, n, cm.data_ptr(), sp)
+        mkgpu.check("mk_cm_poly_design_" + sfx, cm.data_ptr(), n, ord, cm_ord, X.data_ptr(), Xt.data_ptr(), sp)
+        mkgpu.check("mk_design_dots_f64", X.data_ptr(), n, ncol, Xt.data_ptr(), ncol, n, lhs.data_ptr(),
+                    work.data_ptr(), work.numel() * 8, sp)
+        mkgpu.check("mk_design_dots_" + sfx, X.data_ptr(), n, ncol, d.data_ptr(), ndet, n, rhs.data_ptr(),
+                    work.data_ptr(), work.numel() * 8, sp)
+        fitp = np.dot(np.linalg.inv(lhs.cpu().numpy()), rhs.cpu().numpy())
+        coef = torch.from_numpy(np.ascontiguousarray(fitp)).to(d.device)
+        mkgpu.check("mk_design_apply_" + sfx, d.data_ptr(), ndet, n, n, X.data_ptr(), ncol, 0, ord + 1,
+                    coef.data_ptr(), dd.data_ptr(), n, sp)
+        if full_out:
+            pred2 = torch.empty_like(d)
+            coef_cm = coef[ord + 1:]  # rows of a row-major matrix: contiguous
+            mkgpu.check("mk_design_apply_" + sfx, 0, ndet, n, n, X.data_ptr(), ncol, ord + 1, cm_ord,
+                        coef_cm.data_ptr(), pred2.data_ptr(), n, sp)
+    if host:
+        dd, pred2, cm = dd.cpu().numpy(), pred2.cpu().numpy(), cm.cpu().numpy()
+    if full_out:
+        return dd, pred2, cm
+    return dd

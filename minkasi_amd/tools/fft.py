@@ -71,7 +71,8 @@ def set_threaded(n=-1):
 
 
 def read_wisdom(double_file=".fftw_wisdom", single_file=".fftwf_wisdom"):
-    """FFT plans are chosen by the autotuner cache (minkasi_amd.autotune)."""
+    """No FFTW wisdom: the GPU engine picks its transform lengths and tiles
+    from each row length when a plan is built."""
     return None
 
 
