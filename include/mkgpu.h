@@ -334,12 +334,12 @@ int mk_adam_f32(float *p, const float *g, float *m, float *v, int32_t n,
  * Per column c (sample) of a row-major nrow x ncol block (row stride ld):
  * out[c] = np.median over the rows (the mean of the two middle values for an
  * even count); 1 <= nrow <= 1024.  Replaces np.median(dd, axis=0)
- * (processing.py:566). */
+ * (processing.py:568). */
 int mk_median_cols_f32(const float *dat, int64_t nrow, int64_t ncol,
                        int64_t ld, float *out, void *stream);
 int mk_median_cols_f64(const double *dat, int64_t nrow, int64_t ncol,
                        int64_t ld, double *out, void *stream);
-/* Design of the fit (processing.py:556-570): with x_t = (t - (n-1)/2) /
+/* Design of the fit (processing.py:554-571): with x_t = (t - (n-1)/2) /
  * ((n-1)/2), X[t][k] = P_k(x_t) for k <= ord (legvander(xx, ord)) and
  * X[t][ord+1+i] = P_i(x_t) cm[t] for i < cm_ord (cm_pmat * cm); X is n x ncol
  * row-major and Xt its transpose, FP64, ncol = ord + 1 + cm_ord <= 8. */

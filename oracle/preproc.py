@@ -1,7 +1,7 @@
 """Oracle: TOD preprocessing in FP64 (test infrastructure).
 
 Functions, what they follow, and what pins them (tests/test_oracle_preproc.py):
-  legendre_design   -- minkasi/tods/processing.py:553-558 (xx normalised to
+  legendre_design   -- minkasi/tods/processing.py:554-559 (xx normalised to
                        [-1, 1], legvander); the Legendre values by the
                        three-term recurrence.  Pinned by
                        numpy.polynomial.legendre.legvander and the closed
