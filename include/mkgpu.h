@@ -132,11 +132,22 @@ int mk_dot_f32(const float *a, const float *b, int64_t n, double *sc,
                void *work, void *stream);
 int mk_dot_f64(const double *a, const double *b, int64_t n, double *sc,
                void *work, void *stream);
-/* y += a*x with a a host scalar (SkyMap.axpy). */
+/* y += a*x with a a host scalar, evaluated in FP64 (SkyMap.axpy,
+ * skymap.py:428-440; CutsCompact.axpy, cuts.py:445-457). */
 int mk_axpy_f32(float *y, const float *x, double a, int64_t n, void *stream);
-/* out = a*b elementwise (SkyMap.__mul__, Jacobi preconditioner). */
+int mk_axpy_f64(double *y, const double *x, double a, int64_t n,
+                void *stream);
+/* out = a*b elementwise (SkyMap.__mul__ skymap.py:724-740, Jacobi
+ * preconditioner, CutsCompact.__mul__ cuts.py:483-495). */
 int mk_mul_f32(float *out, const float *a, const float *b, int64_t n,
                void *stream);
+int mk_mul_f64(double *out, const double *a, const double *b, int64_t n,
+               void *stream);
+/* y += a*b elementwise (CutsCompact.apply_prior, cuts.py:459-481). */
+int mk_addmul_f32(float *y, const float *a, const float *b, int64_t n,
+                  void *stream);
+int mk_addmul_f64(double *y, const double *a, const double *b, int64_t n,
+                  void *stream);
 /* PCG steps of minkasi/mapmaking/pcg.py:143-170 with device scalars:
  *  mk_pcg_alpha:  pAp = p.Ap ; alpha = zr/pAp  (flag if pAp <= 0 / NaN)
  *  mk_pcg_xr:     x += alpha p ; r -= alpha Ap ; rr = r.r
@@ -368,6 +379,77 @@ int mk_design_apply_f64(const double *base, int64_t nrow, int64_t n,
                         int64_t ld, const double *X, int ncol, int c0, int nk,
                         const double *coef, double *out, int64_t ld_out,
                         void *stream);
+
+/* ---- Cut samples (CutsCompact, minkasi/tods/cuts.py:201-498) -------------
+ * imap: device int64[ncut], flat det * ndata + sample indices into a TOD of
+ * nsamp samples.  Indices outside [0, nsamp) are skipped and counted into
+ * *dev_nbad (device int64, may be NULL).
+ * vec[i] (+)= dat[imap[i]].  Replaces tod2cuts (minkasi.c:260-268). */
+int mk_tod2cuts_f32(float *vec, const float *dat, int64_t nsamp,
+                    const int64_t *imap, int64_t ncut, int do_add,
+                    int64_t *dev_nbad, void *stream);
+int mk_tod2cuts_f64(double *vec, const double *dat, int64_t nsamp,
+                    const int64_t *imap, int64_t ncut, int do_add,
+                    int64_t *dev_nbad, void *stream);
+/* dat[imap[i]] (+)= vec[i].  imap must not repeat an index (the reference's
+ * serial loop would apply repeats in order; here they would race).
+ * Replaces cuts2tod (minkasi.c:271-284). */
+int mk_cuts2tod_f32(float *dat, const float *vec, int64_t nsamp,
+                    const int64_t *imap, int64_t ncut, int do_add,
+                    int64_t *dev_nbad, void *stream);
+int mk_cuts2tod_f64(double *dat, const double *vec, int64_t nsamp,
+                    const int64_t *imap, int64_t ncut, int do_add,
+                    int64_t *dev_nbad, void *stream);
+
+/* ---- Polarised maps (PolMap, minkasi/maps/polmap.py:47-724) --------------
+ * A map holds ncomp values per pixel, index ncomp * ipix + c (the
+ * reference's [nx, ny, npol] array).  twogamma: device array, the
+ * polarisation angle 2 gamma of every sample, in the map's precision.
+ * Pixels outside [0, npix) are skipped and counted into *dev_nbad. */
+#define MK_POL_QU 1         /* (Q, U)                         ncomp 2 */
+#define MK_POL_IQU 2        /* (I, Q, U)                      ncomp 3 */
+#define MK_POL_QU_PRECON 3  /* (QQ, UU, QU)                   ncomp 3 */
+#define MK_POL_IQU_PRECON 4 /* (I, Q, U, QQ, QU, UU)          ncomp 6 */
+/* npol 3: dat (+)= m[3p] + m[3p+1] cos 2g + m[3p+2] sin 2g; npol 2:
+ * dat (+)= m[2p] cos 2g + m[2p+1] sin 2g.  Replaces map2tod_iqu_omp and
+ * map2tod_qu_omp (minkasi.c:146-175) behind polmap2tod
+ * (mapmaking/map2tod.py:57-120). */
+int mk_polmap2tod_f32(float *dat, const float *map, const float *twogamma,
+                      int64_t nsamp, const int32_t *ipix, int64_t npix,
+                      int npol, int do_add, int64_t *dev_nbad, void *stream);
+int mk_polmap2tod_f64(double *dat, const double *map, const double *twogamma,
+                      int64_t nsamp, const int32_t *ipix, int64_t npix,
+                      int npol, int do_add, int64_t *dev_nbad, void *stream);
+/* map += the per-sample terms of mode (MK_POL_*) with hardware atomics
+ * (summation order not fixed).  Replaces tod2map_qu_simple,
+ * tod2map_iqu_simple, tod2map_qu_precon_simple and
+ * tod2map_iqu_precon_simple (minkasi.c:177-257) behind tod2polmap
+ * (mapmaking/tod2map.py:164-215). */
+int mk_tod2polmap_atomic_f32(float *map, const float *dat,
+                             const float *twogamma, int64_t nsamp,
+                             const int32_t *ipix, int64_t npix, int mode,
+                             int64_t *dev_nbad, void *stream);
+int mk_tod2polmap_atomic_f64(double *map, const double *dat,
+                             const double *twogamma, int64_t nsamp,
+                             const int32_t *ipix, int64_t npix, int mode,
+                             int64_t *dev_nbad, void *stream);
+/* In place, every pixel's QU_PRECON 2x2 or IQU_PRECON 3x3 block replaced by
+ * its pseudo-inverse (np.linalg.pinv(block, thresh)), computed in FP64.
+ * IQU_PRECON also zeroes blocks whose determinant is below thresh * (the
+ * map's largest determinant) and sets 1/I where such a block has I > 0.
+ * work: 8 bytes of device memory (IQU_PRECON only; may be NULL for QU).
+ * Replaces PolMap.invert (polmap.py:300-384). */
+int mk_polmap_invert_f32(float *map, int64_t npix, int mode, double thresh,
+                         void *work, int64_t work_bytes, void *stream);
+int mk_polmap_invert_f64(double *map, int64_t npix, int mode, double thresh,
+                         void *work, int64_t work_bytes, void *stream);
+/* out = block(pre) . x per pixel: pre is a QU_PRECON (x, out: QU) or
+ * IQU_PRECON (x, out: IQU) map.  out may equal x.  Replaces
+ * PolMap.__mul__ (polmap.py:624-684). */
+int mk_polmap_precon_mul_f32(float *out, const float *pre, const float *x,
+                             int64_t npix, int mode, void *stream);
+int mk_polmap_precon_mul_f64(double *out, const double *pre, const double *x,
+                             int64_t npix, int mode, void *stream);
 
 #ifdef __cplusplus
 }

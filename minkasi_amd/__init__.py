@@ -1,9 +1,11 @@
 """minkasi_amd -- MI355X-native mapmaking inner loop with minkasi's API.
 
 Public names mirror the reference package (minkasi/__init__.py,
-minkasi/minkasi_all.py): Tod, TodVec, SkyMap, Mapset, NoiseSmoothedSVD,
-run_pcg, run_pcg_wprior, make_hits.  Device work runs in libmkgpu (HIP,
-gfx950); importing the compute modules raises if that library is missing.
+minkasi/minkasi_all.py): Tod, TodVec, SkyMap, PolMap, Mapset,
+NoiseSmoothedSVD, run_pcg, run_pcg_wprior, make_hits, Cuts, CutsCompact,
+tsModel, tsMultiModel, truncate_tod, downsample_tod, fit_cm_plus_poly.
+Device work runs in libmkgpu (HIP, gfx950); importing the compute modules
+raises if that library is missing.
 """
 __version__ = "0.1.0"
 
@@ -28,4 +30,16 @@ def __getattr__(name):
     if name in ("make_hits",):
         from .mapmaking.tod2map import make_hits
         return make_hits
+    if name in ("PolMap",):
+        from .maps.polmap import PolMap
+        return PolMap
+    if name in ("Cuts", "CutsCompact", "segs_from_vec"):
+        from .tods import cuts
+        return getattr(cuts, name)
+    if name in ("tsModel", "tsMultiModel"):
+        from .mapmaking import timestream
+        return getattr(timestream, name)
+    if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens"):
+        from .tods import processing
+        return getattr(processing, name)
     raise AttributeError(name)

@@ -28,6 +28,14 @@ inline unsigned grid_for(int64_t n, int block, int64_t max_groups = 256 * 16) {
   return (unsigned)g;
 }
 
+// Counts lanes with bad == true into *nbad (when nbad is not null): one
+// global atomic per wave that has any such lane.
+__device__ inline void count_bad(bool bad, int64_t* nbad) {
+  unsigned long long m = __ballot(bad);
+  if (nbad && m && (threadIdx.x & 63) == (__ffsll((long long)m) - 1))
+    atomicAdd((unsigned long long*)nbad, (unsigned long long)__popcll(m));
+}
+
 }  // namespace mk
 
 #define MK_REQUIRE(cond, code, ...)      \

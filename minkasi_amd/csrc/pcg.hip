@@ -129,17 +129,52 @@ __global__ void __launch_bounds__(kBlock) stage2(const double* __restrict__ part
   }
 }
 
-__global__ void __launch_bounds__(kBlock) axpy_kernel(float* __restrict__ y, const float* __restrict__ x, double a,
-                                                      int64_t n) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) axpy_kernel(T* __restrict__ y, const T* __restrict__ x, double a, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    y[i] = (float)((double)y[i] + a * (double)x[i]);
+    y[i] = (T)((double)y[i] + a * (double)x[i]);
 }
 
-__global__ void __launch_bounds__(kBlock) mul_kernel(float* __restrict__ o, const float* __restrict__ a,
-                                                     const float* __restrict__ b, int64_t n) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) mul_kernel(T* __restrict__ o, const T* __restrict__ a,
+                                                     const T* __restrict__ b, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) o[i] = a[i] * b[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) addmul_kernel(T* __restrict__ y, const T* __restrict__ a,
+                                                        const T* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T ab = a[i] * b[i];
+    y[i] = y[i] + ab;
+  }
+}
+
+template <typename T>
+int axpy_impl(T* y, const T* x, double a, int64_t n, void* stream) {
+  MK_REQUIRE(n >= 0 && (n == 0 || (x && y)), MK_ERR_ARG, "bad arguments");
+  if (n == 0) return MK_OK;
+  axpy_kernel<T><<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(y, x, a, n);
+  return mk::check_launch("axpy_kernel");
+}
+
+template <typename T>
+int mul_impl(T* out, const T* a, const T* b, int64_t n, void* stream) {
+  MK_REQUIRE(n >= 0 && (n == 0 || (out && a && b)), MK_ERR_ARG, "bad arguments");
+  if (n == 0) return MK_OK;
+  mul_kernel<T><<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(out, a, b, n);
+  return mk::check_launch("mul_kernel");
+}
+
+template <typename T>
+int addmul_impl(T* y, const T* a, const T* b, int64_t n, void* stream) {
+  MK_REQUIRE(n >= 0 && (n == 0 || (y && a && b)), MK_ERR_ARG, "bad arguments");
+  if (n == 0) return MK_OK;
+  addmul_kernel<T><<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(y, a, b, n);
+  return mk::check_launch("addmul_kernel");
 }
 
 __global__ void __launch_bounds__(kBlock) p_kernel(float* __restrict__ p, const float* __restrict__ z,
@@ -186,17 +221,24 @@ extern "C" int mk_dot_f64(const double* a, const double* b, int64_t n, double* s
 }
 
 extern "C" int mk_axpy_f32(float* y, const float* x, double a, int64_t n, void* stream) {
-  MK_REQUIRE(n >= 0 && (n == 0 || (x && y)), MK_ERR_ARG, "bad arguments");
-  if (n == 0) return MK_OK;
-  axpy_kernel<<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(y, x, a, n);
-  return mk::check_launch("axpy_kernel");
+  return axpy_impl<float>(y, x, a, n, stream);
+}
+extern "C" int mk_axpy_f64(double* y, const double* x, double a, int64_t n, void* stream) {
+  return axpy_impl<double>(y, x, a, n, stream);
 }
 
 extern "C" int mk_mul_f32(float* out, const float* a, const float* b, int64_t n, void* stream) {
-  MK_REQUIRE(n >= 0 && (n == 0 || (out && a && b)), MK_ERR_ARG, "bad arguments");
-  if (n == 0) return MK_OK;
-  mul_kernel<<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(out, a, b, n);
-  return mk::check_launch("mul_kernel");
+  return mul_impl<float>(out, a, b, n, stream);
+}
+extern "C" int mk_mul_f64(double* out, const double* a, const double* b, int64_t n, void* stream) {
+  return mul_impl<double>(out, a, b, n, stream);
+}
+
+extern "C" int mk_addmul_f32(float* y, const float* a, const float* b, int64_t n, void* stream) {
+  return addmul_impl<float>(y, a, b, n, stream);
+}
+extern "C" int mk_addmul_f64(double* y, const double* a, const double* b, int64_t n, void* stream) {
+  return addmul_impl<double>(y, a, b, n, stream);
 }
 
 extern "C" int mk_pcg_alpha(const float* p, const float* Ap, int64_t n, double* sc, void* work, void* stream) {

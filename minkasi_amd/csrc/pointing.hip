@@ -98,12 +98,7 @@ __device__ inline int32_t to_ipix(double px, double py, int32_t nx, int32_t ny, 
   return (int32_t)(xp * (double)ny + yp);
 }
 
-__device__ inline void count_bad(bool bad, int64_t* nbad) {
-  // one atomic per wave with any bad lane
-  unsigned long long m = __ballot(bad);
-  if (nbad && m && (threadIdx.x & 63) == (__ffsll((long long)m) - 1))
-    atomicAdd((unsigned long long*)nbad, (unsigned long long)__popcll(m));
-}
+using mk::count_bad;
 
 __global__ void __launch_bounds__(256) pix_kernel(const double* __restrict__ ra, const double* __restrict__ dec,
                                                   int64_t n, CarWcs w, int32_t nx, int32_t ny,

@@ -27,6 +27,9 @@ MK_ERR_ALLOC = 5
 SC_ZR, SC_PAP, SC_ALPHA, SC_BETA, SC_RR, SC_FLAG, SC_DOT = 0, 1, 2, 3, 4, 5, 6
 SC_COUNT = 8
 
+# polarisation modes (MK_POL_* in mkgpu.h)
+POL_QU, POL_IQU, POL_QU_PRECON, POL_IQU_PRECON = 1, 2, 3, 4
+
 
 class MkError(RuntimeError):
     def __init__(self, name, status, msg):
@@ -106,6 +109,22 @@ _SIGS = {
     "mk_design_dots_f64": ([P, I64, I32, P, I64, I64, P, P, I64, P], I),
     "mk_design_apply_f32": ([P, I64, I64, I64, P, I32, I32, I32, P, P, I64, P], I),
     "mk_design_apply_f64": ([P, I64, I64, I64, P, I32, I32, I32, P, P, I64, P], I),
+    "mk_axpy_f64": ([P, P, D, I64, P], I),
+    "mk_mul_f64": ([P, P, P, I64, P], I),
+    "mk_addmul_f32": ([P, P, P, I64, P], I),
+    "mk_addmul_f64": ([P, P, P, I64, P], I),
+    "mk_tod2cuts_f32": ([P, P, I64, P, I64, I, P, P], I),
+    "mk_tod2cuts_f64": ([P, P, I64, P, I64, I, P, P], I),
+    "mk_cuts2tod_f32": ([P, P, I64, P, I64, I, P, P], I),
+    "mk_cuts2tod_f64": ([P, P, I64, P, I64, I, P, P], I),
+    "mk_polmap2tod_f32": ([P, P, P, I64, P, I64, I, I, P, P], I),
+    "mk_polmap2tod_f64": ([P, P, P, I64, P, I64, I, I, P, P], I),
+    "mk_tod2polmap_atomic_f32": ([P, P, P, I64, P, I64, I, P, P], I),
+    "mk_tod2polmap_atomic_f64": ([P, P, P, I64, P, I64, I, P, P], I),
+    "mk_polmap_invert_f32": ([P, I64, I, D, P, I64, P], I),
+    "mk_polmap_invert_f64": ([P, I64, I, D, P, I64, P], I),
+    "mk_polmap_precon_mul_f32": ([P, P, P, I64, I, P], I),
+    "mk_polmap_precon_mul_f64": ([P, P, P, I64, I, P], I),
 }
 
 _fns = {}

@@ -31,6 +31,8 @@ class BatchedDot:
         from ..maps.skymap import SkyMap
         if len(mapset.maps) != 1 or not isinstance(mapset.maps[0], SkyMap):
             return False
+        if getattr(mapset.maps[0], "npol", 1) != 1:
+            return False
         if mapset.maps[0].map.dtype != torch.float32 or todvec.ntod == 0:
             return False
         for tod in todvec.tods:

@@ -14,3 +14,28 @@ def map2tod(dat, map, ipix, do_add=False, do_omp=True):
         raise ValueError("dat has %d samples, ipix %d" % (dat.numel(), ipix.numel()))
     mkgpu.check(name, dev.ptr(dat, name="dat"), dev.ptr(map, name="map"), dat.numel(),
                 dev.ptr(ipix, torch.int32, "ipix"), int(bool(do_add)), dev.stream_ptr())
+
+
+def polmap2tod(dat, map, poltag, twogamma, ipix, do_add=False, do_omp=True, npix=None):
+    """Unroll a QU or IQU map into a TOD in place (mapmaking/map2tod.py:57-120):
+    dat (+)= [I +] Q cos 2g + U sin 2g.  map is (nx, ny, npol); npix defaults
+    to nx * ny.  Samples whose pixel falls outside the map raise."""
+    npol = {"QU": 2, "IQU": 3}.get(poltag)
+    if npol is None:
+        raise ValueError("unrecognized poltag " + repr(poltag) + " in polmap2tod.")
+    if dat.dtype != map.dtype or twogamma.dtype != map.dtype:
+        raise TypeError("dat, map and twogamma must share a dtype")
+    if dat.numel() != ipix.numel() or twogamma.numel() != dat.numel():
+        raise ValueError("dat, twogamma and ipix differ in size")
+    if npix is None:
+        npix = map.numel() // npol
+    if map.numel() < npix * npol:
+        raise ValueError("map holds fewer than npix * npol values")
+    name = {torch.float32: "mk_polmap2tod_f32", torch.float64: "mk_polmap2tod_f64"}[dat.dtype]
+    nbad = torch.zeros(1, dtype=torch.int64, device=dat.device)
+    mkgpu.check(name, dev.ptr(dat, name="dat"), dev.ptr(map, name="map"), dev.ptr(twogamma, name="twogamma"),
+                dat.numel(), dev.ptr(ipix, torch.int32, "ipix"), npix, npol, int(bool(do_add)), nbad.data_ptr(),
+                dev.stream_ptr())
+    nb = int(nbad.item())
+    if nb:
+        raise ValueError("%d samples fall outside the map in polmap2tod" % nb)

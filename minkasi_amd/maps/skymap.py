@@ -119,7 +119,7 @@ class SkyMap:
         return copy.deepcopy(self)
 
     def __deepcopy__(self, memo):
-        new = SkyMap.__new__(SkyMap)
+        new = self.__class__.__new__(self.__class__)
         for k, v in self.__dict__.items():
             if k == "map":
                 new.map = v.clone()
@@ -132,11 +132,10 @@ class SkyMap:
 
     def axpy(self, map, a):
         """self.map += a * map.map (skymap.py:428-440)."""
-        if self.map.dtype == torch.float32 and map.map.dtype == torch.float32:
-            mkgpu.check("mk_axpy_f32", self.map.data_ptr(), map.map.data_ptr(), float(a), self.map.numel(),
-                        dev.stream_ptr())
-        else:
-            self.map.add_(map.map.to(self.map.dtype), alpha=float(a))
+        sfx = {torch.float32: "f32", torch.float64: "f64"}[self.map.dtype]
+        x = map.map.to(self.map.dtype).contiguous()
+        mkgpu.check("mk_axpy_" + sfx, self.map.data_ptr(), x.data_ptr(), float(a), self.map.numel(),
+                    dev.stream_ptr())
 
     def assign(self, arr):
         assert arr.shape[0] == self.nx
@@ -160,11 +159,10 @@ class SkyMap:
     def __mul__(self, map):
         """skymap.py:724-740: elementwise product in a copy of `map`."""
         new_map = map.copy()
-        if self.map.dtype == torch.float32 and map.map.dtype == torch.float32:
-            mkgpu.check("mk_mul_f32", new_map.map.data_ptr(), self.map.data_ptr(), map.map.data_ptr(),
-                        self.map.numel(), dev.stream_ptr())
-        else:
-            new_map.map.copy_(self.map.to(new_map.map.dtype) * map.map)
+        sfx = {torch.float32: "f32", torch.float64: "f64"}[new_map.map.dtype]
+        a = self.map.to(new_map.map.dtype).contiguous()
+        mkgpu.check("mk_mul_" + sfx, new_map.map.data_ptr(), a.data_ptr(), map.map.data_ptr(),
+                    self.map.numel(), dev.stream_ptr())
         return new_map
 
     def invert(self):
@@ -270,7 +268,10 @@ class SkyMap:
     # ------------------------------------------------------------------ output
     def write(self, fname="map.fits"):
         """skymap.py:705-722: FITS primary HDU of map.T with the WCS cards."""
-        data = self.map.to(torch.float64).cpu().numpy().T.copy()
+        self._write_fits(fname, self.map)
+
+    def _write_fits(self, fname, plane):
+        data = plane.to(torch.float64).cpu().numpy().T.copy()
         cards = [("SIMPLE", True), ("BITPIX", -64), ("NAXIS", 2), ("NAXIS1", data.shape[1]),
                  ("NAXIS2", data.shape[0])] + self.wcs.to_header()
         fname = os.path.abspath(fname)

@@ -204,6 +204,22 @@ class Tod:
         filt = self.apply_noise(d)
         return float((filt.to(torch.float64) * d.to(torch.float64)).sum().item())
 
+    def prior_from_skymap(self, skymap):
+        """A CutsCompact prior on the samples where skymap is non-zero, with
+        weight 1/value^2 (core.py:751-782)."""
+        from .cuts import CutsCompact
+        tmp = self.get_empty(True).to(skymap.map.dtype)
+        skymap.map2tod(self, tmp)
+        mask = tmp == 0
+        prior = CutsCompact(self, dtype=skymap.map.dtype)
+        prior.cuts_from_array(mask)
+        prior.get_imap()
+        prior.tod2map(self, tmp)
+        if prior.map is None:
+            raise ValueError("Prior map is None, something went wrong")
+        prior.map = 1.0 / prior.map ** 2
+        return prior
+
 
 class TodVec:
     def __init__(self):

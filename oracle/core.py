@@ -9,6 +9,13 @@ Functions and what pins them (tests/test_oracle.py):
                              own minkasi.c (oracle/_ref).  minkasi.c:116-144.
   tod2map                 -- np.bincount and the reference's tod2map_simple
                              (oracle/_ref).  minkasi.c:50-55, 97-112.
+  tod2cuts / cuts2tod     -- closed forms (fancy indexing) and the reference's
+                             own tod2cuts / cuts2tod (oracle/_ref).
+                             minkasi.c:260-284.
+  polmap2tod / tod2polmap -- closed forms (cos/sin-weighted gathers and
+                             np.bincount) and the reference's own
+                             map2tod_{iqu,qu}_omp, tod2map_*_simple
+                             (oracle/_ref).  minkasi.c:146-257.
 """
 import ctypes
 import os
@@ -38,6 +45,10 @@ def lib():
         L.or_car_world2pix.argtypes = [P, P, I64, P, P, P, P]
         L.or_pix_from_radec.argtypes = [P, P, I64, P, P, ctypes.c_int32, P, I]
         L.or_dct1_direct.argtypes = [P, I64, I64, P]
+        L.or_tod2cuts.argtypes = [P, P, P, I64, I]
+        L.or_cuts2tod.argtypes = [P, P, P, I64, I]
+        L.or_polmap2tod.argtypes = [P, P, P, I64, P, I, I]
+        L.or_tod2polmap.argtypes = [P, P, P, I64, P, I]
         _lib = L
     return _lib
 
@@ -58,6 +69,14 @@ def ref_lib():
         L.tod2map_simple.argtypes = [P, P, I, I, P]
         L.tod2map_omp.argtypes = [P, P, I, I, P, I]
         L.set_nthread.argtypes = [I]
+        L.tod2cuts.argtypes = [P, P, P, I, I]
+        L.cuts2tod.argtypes = [P, P, P, I, I]
+        L.map2tod_iqu_omp.argtypes = [P, P, P, I, I, P, I]
+        L.map2tod_qu_omp.argtypes = [P, P, P, I, I, P, I]
+        L.tod2map_qu_simple.argtypes = [P, P, P, I, I, P]
+        L.tod2map_iqu_simple.argtypes = [P, P, P, I, I, P]
+        L.tod2map_qu_precon_simple.argtypes = [P, P, P, I, I, P]
+        L.tod2map_iqu_precon_simple.argtypes = [P, P, P, I, I, P]
         _ref = L
     return _ref
 
@@ -172,3 +191,35 @@ def dct1_direct(x):
     out = np.empty_like(x)
     lib().or_dct1_direct(_p(x), x.shape[0], x.shape[1], _p(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# Cut samples and polarised projections (minkasi.c:146-284)
+
+POL_NCOMP = {1: 2, 2: 3, 3: 3, 4: 6}
+
+
+def tod2cuts(vec, dat, imap, do_add=True):
+    """vec (+)= dat.ravel()[imap], serially (minkasi.c:260-268)."""
+    assert vec.dtype == np.float64 and dat.dtype == np.float64 and imap.dtype == np.int64
+    assert dat.flags.c_contiguous and vec.flags.c_contiguous
+    lib().or_tod2cuts(_p(vec), _p(dat), _p(imap), len(imap), int(bool(do_add)))
+
+
+def cuts2tod(dat, vec, imap, do_add=True):
+    """dat.ravel()[imap] (+)= vec, serially (minkasi.c:271-284)."""
+    assert vec.dtype == np.float64 and dat.dtype == np.float64 and imap.dtype == np.int64
+    assert dat.flags.c_contiguous and vec.flags.c_contiguous
+    lib().or_cuts2tod(_p(dat), _p(vec), _p(imap), len(imap), int(bool(do_add)))
+
+
+def polmap2tod(dat, map_, twogamma, ipix, npol, do_add=True):
+    """npol 3 (IQU) or 2 (QU), map_ pixel-major (npix * npol)."""
+    assert dat.dtype == map_.dtype == twogamma.dtype == np.float64 and ipix.dtype == np.int32
+    lib().or_polmap2tod(_p(dat), _p(map_), _p(twogamma), dat.size, _p(ipix), int(npol), int(bool(do_add)))
+
+
+def tod2polmap(map_, dat, twogamma, ipix, mode):
+    """mode 1 QU, 2 IQU, 3 QU_PRECON, 4 IQU_PRECON; map_ += serially."""
+    assert dat.dtype == map_.dtype == twogamma.dtype == np.float64 and ipix.dtype == np.int32
+    lib().or_tod2polmap(_p(map_), _p(dat), _p(twogamma), dat.size, _p(ipix), int(mode))

@@ -12,6 +12,13 @@
  *   or_car_*          -- astropy 4.3.1 / wcslib 7.6 fixtures in tests/golden.
  *   or_dct1_direct    -- brute-force DCT-I definition (REDFT00), pinned against
  *                        scipy.fft.dct(type=1) and DCT-I^2 = 2(n-1) I.
+ *   or_tod2cuts,      -- closed forms vec = dat.ravel()[imap] and
+ *   or_cuts2tod          dat.ravel()[imap] = vec; bit-equal to the reference's
+ *                        tod2cuts / cuts2tod built into oracle/_ref.
+ *   or_polmap2tod,    -- closed forms m[3p] + m[3p+1] cos + m[3p+2] sin and
+ *   or_tod2polmap        np.bincount with the cos/sin weights; bit-equal to the
+ *                        reference's map2tod_{iqu,qu}_omp and
+ *                        tod2map_{qu,iqu}{,_precon}_simple built into oracle/_ref.
  */
 #include <math.h>
 #include <stdint.h>
@@ -264,6 +271,57 @@ void or_dct1_direct(const double *x, int64_t nrow, int64_t n, double *out)
         s += 2.0 * xr[j] * cos(OR_PI * (double)(2 * ph) / (double)N);
       }
       yr[k] = s;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Cut samples (CutsCompact, minkasi/tods/cuts.py:201-498).  Follows
+ * minkasi.c:260-284 (tod2cuts, cuts2tod): serial loops in index order.     */
+void or_tod2cuts(double *vec, const double *dat, const int64_t *imap,
+                 int64_t ncut, int do_add)
+{
+  for (int64_t i = 0; i < ncut; i++)
+    vec[i] = do_add ? vec[i] + dat[imap[i]] : dat[imap[i]];
+}
+
+void or_cuts2tod(double *dat, const double *vec, const int64_t *imap,
+                 int64_t ncut, int do_add)
+{
+  for (int64_t i = 0; i < ncut; i++)
+    dat[imap[i]] = do_add ? dat[imap[i]] + vec[i] : vec[i];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Polarised projections (PolMap, minkasi/maps/polmap.py:464-545).  Follows
+ * minkasi.c:146-175 (map2tod_iqu_omp, map2tod_qu_omp) and :177-257
+ * (tod2map_{qu,iqu}_simple, tod2map_{qu,iqu}_precon_simple).  mode: 1 QU,
+ * 2 IQU, 3 QU_PRECON, 4 IQU_PRECON; ncomp values per pixel, pixel-major.   */
+void or_polmap2tod(double *dat, const double *map, const double *twogamma,
+                   int64_t nn, const int32_t *pix, int npol, int do_add)
+{
+  for (int64_t i = 0; i < nn; i++) {
+    const double c = cos(twogamma[i]), s = sin(twogamma[i]);
+    const double *m = map + (int64_t)npol * pix[i];
+    const double v = npol == 3 ? m[0] + m[1] * c + m[2] * s : m[0] * c + m[1] * s;
+    dat[i] = do_add ? dat[i] + v : v;
+  }
+}
+
+void or_tod2polmap(double *map, const double *dat, const double *twogamma,
+                   int64_t nn, const int32_t *pix, int mode)
+{
+  const int nc = mode == 1 ? 2 : mode == 2 ? 3 : mode == 3 ? 3 : 6;
+  for (int64_t i = 0; i < nn; i++) {
+    const double c = cos(twogamma[i]), s = sin(twogamma[i]), d = dat[i];
+    double *m = map + (int64_t)nc * pix[i];
+    switch (mode) {
+      case 1: m[0] += d * c; m[1] += d * s; break;
+      case 2: m[0] += d; m[1] += d * c; m[2] += d * s; break;
+      case 3: m[0] += d * c * c; m[1] += d * s * s; m[2] += d * s * c; break;
+      default:
+        m[0] += d; m[1] += d * c; m[2] += d * s;
+        m[3] += d * c * c; m[4] += d * s * c; m[5] += d * s * s;
     }
   }
 }
