@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default plans instead of the autotuner's (FFT column lengths, tod2map tiles)")
     ap.add_argument("--truncate", action="store_true",
                     help="cut every TOD so that n-1 is {2,3,5,7,11}-smooth first (truncate_tod)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3: setup + steps only")
@@ -156,6 +158,13 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t_setup = time.time()
+    tuner, t_tune = None, 0.0
+    if not args.no_tune:
+        from minkasi_amd import tune
+        t_tune = time.time()
+        tuner = tune.enable()
+        t_tune = time.time() - t_tune
+        log("tune: FFT costs ready in %.1fs (cache %s)" % (t_tune, tuner.path))
     P = build_problem(args.config, ntod=args.ntod, verbose=True, truncate=args.truncate)
     log("setup: generate %.1fs pixelize %.1fs noise %.1fs plans %.1fs rhs %.1fs; %d TODs, %.3g samples" % (
         P.t_generate, P.t_pixelize, P.t_noise, P.t_plans, P.t_rhs, len(P.shapes), P.nsamp))
@@ -210,6 +219,8 @@ def main():
         "setup_s": round(t_setup, 1),
         "roofline": roof,
     }
+    if tuner is not None:
+        result["config"]["tuning"] = dict(tuner.report(), fft_measure_s=round(t_tune, 1))
     if args.profile:
         print(json.dumps(result))
         return

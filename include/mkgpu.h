@@ -190,7 +190,9 @@ int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
  * DCT-I / weight / DCT-I core of NoiseSmoothedSVD.apply_noise
  * (minkasi/mapmaking/noise.py:705-708), batched over every mode of every
  * TOD.  Computed as one circular convolution per row through a real FFT of
- * L = 2^q >= 3(n-1)+1 points (exact for any n).
+ * L >= 3(n-1)+1 points (exact for any n): L = 2^11 P1 with P1 5-smooth in
+ * [2, 256] on the FP32 path (by default the smallest such L), else
+ * L = 2^q.
  * desc (host): 6 int64 per segment (TOD): rows, n, in_off, in_stride,
  *   out_off, out_stride (element offsets into the exec in/out arrays).
  * elem_bytes: 4 (FP32 transform) or 8 (FP64 transform).
@@ -201,6 +203,17 @@ int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
 typedef struct mk_filt_plan mk_filt_plan;
 int mk_filt_plan_create(int nseg, const int64_t *desc, int elem_bytes,
                         mk_filt_plan **plan);
+/* As mk_filt_plan_create, with the column length P1 of every segment chosen
+ * by the caller (an autotuner; the FFTW-wisdom counterpart of
+ * minkasi/tools/fft.py:347-384): p1[i] = 0 keeps the default, otherwise it
+ * must be one of the FP32 hot lengths (mk_filt_hot_p1) with
+ * 2^11 p1[i] >= the segment's minimum length.  Plans whose spectra are
+ * borrowed must agree on P1. */
+int mk_filt_plan_create_p1(int nseg, const int64_t *desc, int elem_bytes,
+                           const int32_t *p1, mk_filt_plan **plan);
+/* Writes up to cap hot column lengths (ascending) to out (may be NULL) and
+ * returns how many there are. */
+int mk_filt_hot_p1(int32_t *out, int cap);
 /* wdct: device FP64, rows of C W (the DCT-I of each row of mywt), laid out
  * exactly like the exec input (in_off/in_stride of desc). */
 int mk_filt_plan_set_weights(mk_filt_plan *plan, const double *wdct,

@@ -1164,19 +1164,21 @@ struct Geometry {
 // mixed: the FP32 hot path takes any 5-smooth P1 in [2, 256] with P2 = 2^11,
 // so P = P1 2^11 is the smallest such multiple of 2^11 >= P_min (instead of
 // the next power of two): about 1.4x fewer points on average.
+Geometry hot_geometry(int p1) {
+  Geometry g;
+  g.p1 = p1;
+  g.l1 = (p1 & (p1 - 1)) == 0 ? ilog2c(p1) : -1;
+  g.l2 = kHotL2;
+  g.P = (int64_t)p1 * kHotP2;
+  g.cw = generic_cw(p1);
+  return g;
+}
+
 Geometry geometry(int64_t P_min, int elem_bytes, bool mixed = false) {
   if (mixed && P_min > kHotP2) {
     const int64_t need = (P_min + kHotP2 - 1) / kHotP2;
     for (int i = 0; i < kNumHotP1; i++)
-      if (kHotP1[i] >= need) {
-        Geometry g;
-        g.p1 = kHotP1[i];
-        g.l1 = (g.p1 & (g.p1 - 1)) == 0 ? ilog2c(g.p1) : -1;
-        g.l2 = kHotL2;
-        g.P = (int64_t)g.p1 * kHotP2;
-        g.cw = generic_cw(g.p1);
-        return g;
-      }
+      if (kHotP1[i] >= need) return hot_geometry(kHotP1[i]);
   }
   int q = std::max(2, log2_ceil(P_min));
   static int l2max = [] {
@@ -1431,8 +1433,11 @@ void free_filt(mk_filt_plan* p) {
 // desc per segment: {rows, n, in_off, in_stride, out_off, out_stride}
 // mixed: FP32 filter plans whose rows may take the hot path's 5-smooth
 // column lengths.
+// p1 (may be null): per segment a hot column length chosen by the caller
+// (0 = the default, the smallest one that fits).
 int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<Seg>& segs, int64_t* n1,
-               int64_t* n2, int64_t* t_elems, int64_t* s_elems, size_t* lds1, size_t* lds2, bool mixed = false) {
+               int64_t* n2, int64_t* t_elems, int64_t* s_elems, size_t* lds1, size_t* lds2, bool mixed = false,
+               const int32_t* p1 = nullptr) {
   segs.clear();
   int64_t b1 = 0, b2 = 0, toff = 0, soff = 0;
   size_t l1max = 0, l2max = 0;
@@ -1443,7 +1448,15 @@ int build_segs(int nseg, const int64_t* desc, int eb, bool filter, std::vector<S
                "segment %d: need rows >= 0 and 2 <= n < 2^30 (rows=%lld n=%lld)", i, (long long)rows,
                (long long)n);
     int64_t m = n - 1;
-    Geometry g = geometry(filter ? (3 * m + 2) / 2 : 2 * m + 1, eb, mixed);
+    const int64_t P_min = filter ? (3 * m + 2) / 2 : 2 * m + 1;
+    Geometry g = geometry(P_min, eb, mixed);
+    if (p1 && p1[i] > 0) {
+      MK_REQUIRE(mixed && hot_p1_index(p1[i]) >= 0, MK_ERR_ARG,
+                 "segment %d: column length %d is not one of the FP32 hot lengths", i, (int)p1[i]);
+      MK_REQUIRE((int64_t)p1[i] * kHotP2 >= P_min && P_min > kHotP2, MK_ERR_ARG,
+                 "segment %d: column length %d is too short for n=%lld", i, (int)p1[i], (long long)n);
+      g = hot_geometry(p1[i]);
+    }
     MK_REQUIRE(g.P <= (int64_t(1) << 24), MK_ERR_ARG, "segment %d: n=%lld too long", i, (long long)n);
     Seg s{};
     s.in_off = d[2]; s.in_stride = d[3]; s.out_off = d[4]; s.out_stride = d[5];
@@ -1534,6 +1547,17 @@ void launch_filter(const mk_filt_plan* p, const TIO* in, TIO* out, hipStream_t s
 }  // namespace
 
 extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes, mk_filt_plan** out) {
+  return mk_filt_plan_create_p1(nseg, desc, elem_bytes, nullptr, out);
+}
+
+extern "C" int mk_filt_hot_p1(int32_t* out, int cap) {
+  if (out)
+    for (int i = 0; i < kNumHotP1 && i < cap; i++) out[i] = kHotP1[i];
+  return kNumHotP1;
+}
+
+extern "C" int mk_filt_plan_create_p1(int nseg, const int64_t* desc, int elem_bytes, const int32_t* p1,
+                                      mk_filt_plan** out) {
   MK_REQUIRE(out, MK_ERR_ARG, "out is NULL");
   *out = nullptr;
   MK_REQUIRE(nseg >= 0 && (nseg == 0 || desc), MK_ERR_ARG, "bad segment table");
@@ -1545,7 +1569,7 @@ extern "C" int mk_filt_plan_create(int nseg, const int64_t* desc, int elem_bytes
   p->elem_bytes = elem_bytes;
   const bool hot = elem_bytes == 4 && getenv("MK_FFT_NO_HOT") == nullptr;
   int st = build_segs(nseg, desc, elem_bytes, true, p->h_segs, &p->n1, &p->n2, &p->t_elems, &p->s_elems, &p->lds1,
-                      &p->lds2, hot);
+                      &p->lds2, hot, p1);
   if (st) { free_filt(p); return st; }
   MK_REQUIRE(p->n1 < INT32_MAX && p->n2 < INT32_MAX, (free_filt(p), MK_ERR_ARG), "too many work items");
   if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||

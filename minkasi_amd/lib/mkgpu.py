@@ -113,6 +113,8 @@ _SIGS = {
     "mk_mul_f64": ([P, P, P, I64, P], I),
     "mk_addmul_f32": ([P, P, P, I64, P], I),
     "mk_addmul_f64": ([P, P, P, I64, P], I),
+    "mk_filt_plan_create_p1": ([I, P, I, P, P], I),
+    "mk_filt_hot_p1": ([P, I], I),
     "mk_tod2cuts_f32": ([P, P, I64, P, I64, I, P, P], I),
     "mk_tod2cuts_f64": ([P, P, I64, P, I64, I, P, P], I),
     "mk_cuts2tod_f32": ([P, P, I64, P, I64, I, P, P], I),

@@ -69,15 +69,30 @@ class FiltPlan(_Handle):
 
     _destroy = "mk_filt_plan_destroy"
 
-    def __init__(self, segs, elem_bytes=4):
-        """segs: list of (rows, n, in_off, in_stride, out_off, out_stride)."""
+    def __init__(self, segs, elem_bytes=4, p1=None):
+        """segs: list of (rows, n, in_off, in_stride, out_off, out_stride).
+        p1: per segment the FFT column length (0 = the smallest that fits);
+        None asks the active autotuner (minkasi_amd.tune), if any."""
         super().__init__()
         self.segs = [tuple(int(v) for v in s) for s in segs]
         self.elem_bytes = elem_bytes
         desc = np.asarray(self.segs, dtype=np.int64).reshape(-1)
         if desc.size == 0:
             desc = np.zeros(6, dtype=np.int64)
-        mkgpu.check("mk_filt_plan_create", len(self.segs), desc.ctypes.data, int(elem_bytes), ctypes.byref(self.h))
+        if p1 is None and elem_bytes == 4:
+            from .. import tune
+            t = tune.active()
+            if t is not None:
+                p1 = [t.fft_p1(s[1]) for s in self.segs]
+        if p1 is None:
+            mkgpu.check("mk_filt_plan_create", len(self.segs), desc.ctypes.data, int(elem_bytes),
+                        ctypes.byref(self.h))
+        else:
+            p = np.asarray(list(p1) + [0], dtype=np.int32)
+            if len(p) != len(self.segs) + 1:
+                raise ValueError("p1 needs one entry per segment")
+            mkgpu.check("mk_filt_plan_create_p1", len(self.segs), desc.ctypes.data, int(elem_bytes), p.ctypes.data,
+                        ctypes.byref(self.h))
 
     def set_weights(self, wdct):
         mkgpu.check("mk_filt_plan_set_weights", self.h, dev.ptr(wdct, torch.float64, "wdct"), dev.stream_ptr())

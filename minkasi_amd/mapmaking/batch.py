@@ -14,6 +14,7 @@ stage is ONE launch over the whole arena:
 
 The sequence can be captured once in a HIP graph and replayed.
 """
+import os
 import time
 
 import numpy as np
@@ -23,6 +24,13 @@ from .. import device as dev
 from ..lib.plans import FiltPlan, RotPlan, T2MPlan
 from .map2tod import map2tod
 from .noise import NoiseSmoothedSVD
+
+
+def _hot_p1(plan):
+    """The FFT column length of a one-segment FP32 filter plan when it is on
+    the hot path (rows of 2^11 points, mixed-radix columns), else 0."""
+    g = plan.geometry()[0]
+    return int(g[0]) if int(g[1]) == 2048 and not os.environ.get("MK_FFT_NO_HOT") else 0
 
 
 class BatchedDot:
@@ -63,11 +71,17 @@ class BatchedDot:
             self.ipix[o:o + ndet * n].copy_(ip.reshape(-1))
             tod.info[self.tag] = self.ipix[o:o + ndet * n].view(ndet, n)
         self.ipix_ids = [id(tod.info[self.tag]) for tod in self.tods]
-        self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
         self.buf_a = torch.empty(max(off, 4), dtype=torch.float32, device=d)
         self.buf_b = torch.empty(max(off, 4), dtype=torch.float32, device=d)
+        from .. import tune
+        tuner = tune.active()
+        if tuner is not None and off > 0 and tile == (64, 32) and item_samples is None:
+            self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
+        else:
+            self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
         segs = [(ndet, n, o, n, o, n) for o, (ndet, n) in zip(offs, sizes)]
-        self.filt = FiltPlan(segs, 4)
+        # the same FFT column lengths as the noise models whose spectra are borrowed
+        self.filt = FiltPlan(segs, 4, p1=[_hot_p1(tod.noise.filt32) for tod in self.tods])
         for i, tod in enumerate(self.tods):
             self.filt.borrow(i, tod.noise.filt32, 0)
         self.filt.set_halve(True)

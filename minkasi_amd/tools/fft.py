@@ -71,10 +71,19 @@ def set_threaded(n=-1):
 
 
 def read_wisdom(double_file=".fftw_wisdom", single_file=".fftwf_wisdom"):
-    """No FFTW wisdom: the GPU engine picks its transform lengths and tiles
-    from each row length when a plan is built."""
-    return None
+    """Counterpart of loading FFTW wisdom (minkasi/tools/fft.py:345-364):
+    enables the autotuner (minkasi_amd.tune) from its cache, measuring the
+    FFT column lengths once if the cache has none for this device.  The file
+    names are FFTW's and are not used; the cache lives at
+    minkasi_amd.tune.cache_path()."""
+    from .. import tune
+    return tune.enable()
 
 
 def write_wisdom(double_file=".fftw_wisdom", single_file=".fftwf_wisdom"):
+    """Saves the active autotuner's cache (minkasi/tools/fft.py:367-384)."""
+    from .. import tune
+    t = tune.active()
+    if t is not None:
+        t.save()
     return None

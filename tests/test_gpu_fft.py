@@ -60,9 +60,10 @@ def test_dct1_involution_large(env):
     assert rel_err(y / (2 * (n - 1)), x) < 1e-12
 
 
-def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False, geometry=None):
+def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False, geometry=None, p1=None):
     """Batched filter over segments of (rows, n); returns gpu, ref, info.
-    geometry: optional list that receives each segment's (P1, P2, cw, P)."""
+    geometry: optional list that receives each segment's (P1, P2, cw, P).
+    p1: optional per-segment column lengths (mk_filt_plan_create_p1)."""
     torch, device, mkgpu = env
     rng = np.random.default_rng(seed)
     desc = []
@@ -76,7 +77,11 @@ def _filter(env, rows_n, dtype, halve, seed=0, wdct_from_gpu=False, geometry=Non
     desc = np.asarray(desc, dtype=np.int64)
     eb = 8 if dtype == np.float64 else 4
     h = ctypes.c_void_p()
-    mkgpu.check("mk_filt_plan_create", len(rows_n), desc.ctypes.data, eb, ctypes.byref(h))
+    if p1 is None:
+        mkgpu.check("mk_filt_plan_create", len(rows_n), desc.ctypes.data, eb, ctypes.byref(h))
+    else:
+        p1a = np.asarray(p1, dtype=np.int32)
+        mkgpu.check("mk_filt_plan_create_p1", len(rows_n), desc.ctypes.data, eb, p1a.ctypes.data, ctypes.byref(h))
     try:
         if wdct_from_gpu:
             wd = np.concatenate([gpu_dct1(env, w, np.float64).ravel() for w in ws])
@@ -204,3 +209,32 @@ def test_filter_fp32_every_hot_column_length(env):
         assert rel_err(o, r) < 5e-6, (p1, n, rel_err(o, r))
         assert rel_err(o, g) < 5e-6, (p1, n)
         off += n
+
+
+def test_filter_fp32_longer_column_lengths_are_exact(env):
+    """Rows filtered with a caller-chosen P1 above the smallest that fits
+    (what the autotuner may pick) still equal the FP64 DCT-I pair; a P1 that
+    is too short or not a hot length is refused."""
+    p1s = [v for v in range(2, 257) if _smooth(v)]
+    segs, choice = [], []
+    for n in (2800, 9000, 65536, 100001, 200000, 349000):
+        need = -(-((3 * (n - 1) + 2) // 2) // 2048)
+        fits = [p for p in p1s if p >= need]
+        for p in fits[:3] + fits[-1:]:
+            segs.append((2, n))
+            choice.append(p)
+    geo = []
+    out, ref, _ = _filter(env, segs, np.float32, halve=True, seed=13, geometry=geo, p1=choice)
+    assert [g[0] for g in geo] == choice
+    off = 0
+    for (rows, n), p in zip(segs, choice):
+        o, r = out[off:off + rows * n], ref[off:off + rows * n]
+        assert rel_err(o, r) < 5e-6, (p, n, rel_err(o, r))
+        off += rows * n
+    torch, device, mkgpu = env
+    h = ctypes.c_void_p()
+    for bad in (3, 7):  # 3 is too short for n = 65536; 7 is not 5-smooth
+        desc = np.asarray([1, 65536, 0, 65536, 0, 65536], dtype=np.int64)
+        p = np.asarray([bad], dtype=np.int32)
+        with pytest.raises(mkgpu.MkError):
+            mkgpu.check("mk_filt_plan_create_p1", 1, desc.ctypes.data, 4, p.ctypes.data, ctypes.byref(h))
