@@ -79,6 +79,30 @@ int mk_map2tod_f32(float *dat, const float *map, int64_t nsamp,
 int mk_map2tod_f64(double *dat, const double *map, int64_t nsamp,
                    const int32_t *ipix, int do_add, void *stream);
 
+/* Tiled map layout for the map2tod gather (an MI355X layout; the
+ * reference's map2tod_omp, minkasi.c:131-144, gathers from the row-major
+ * map).  Pixel (x, y) of an nx x ny map is stored at
+ *   ((x / tx) * ceil(ny / ty) + y / ty) * tx * ty + (x % tx) * ty + y % ty,
+ * so a raster track touches a few 128-byte lines per wave instead of one
+ * per map row.  mk_map_tiled_size gives the tiled array's length (-1 on bad
+ * sizes); tx * ty <= 1024.  mk_pix_tile_remap rewrites row-major indices
+ * (x * ny + y) into tiled ones; indices outside the map are counted into
+ * *dev_nbad (device int64, may be NULL) and written as 0.
+ * mk_map_to_tiles_* copies a row-major map into the tiled array (padding
+ * zero).  mk_map2tod_* on (tiled map, remapped indices) equals mk_map2tod_*
+ * on (map, indices) bit for bit. */
+int64_t mk_map_tiled_size(int32_t nx, int32_t ny, int32_t tile_x,
+                          int32_t tile_y);
+int mk_pix_tile_remap(const int32_t *ipix, int64_t nsamp, int32_t nx,
+                      int32_t ny, int32_t tile_x, int32_t tile_y,
+                      int32_t *out, int64_t *dev_nbad, void *stream);
+int mk_map_to_tiles_f32(const float *map, int32_t nx, int32_t ny,
+                        int32_t tile_x, int32_t tile_y, float *tiled,
+                        void *stream);
+int mk_map_to_tiles_f64(const double *map, int32_t nx, int32_t ny,
+                        int32_t tile_x, int32_t tile_y, double *tiled,
+                        void *stream);
+
 /* tod2map with global atomics: map[ipix[i]] += dat[i].
  * Counterpart of tod2map_atomic (minkasi.c:59-66); order-nondeterministic
  * like the reference's tod2map_omp/tod2map_atomic.  Kept as the

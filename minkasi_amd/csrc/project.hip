@@ -99,6 +99,62 @@ int map2tod_impl(T* dat, const T* map, int64_t nsamp, const int32_t* ipix, int d
   return mk::check_launch("map2tod_kernel");
 }
 
+// ---------------------------------------------------------------------------
+// Tiled map layout for the gather.  A raster scan moves along one map axis,
+// so 64 consecutive samples of one detector fall on ~20 rows of a row-major
+// map (stride ny) and a wave's gathers touch ~20 cache lines.  With the map
+// re-laid in tx x ty pixel tiles (128 bytes for 32 FP32 pixels) the same
+// samples touch a few lines:
+//   tiled(x, y) = ((x / tx) * nty + y / ty) * (tx * ty) + (x % tx) * ty + y % ty,
+//   nty = ceil(ny / ty).
+
+__global__ void __launch_bounds__(256) pix_tile_kernel(const int32_t* __restrict__ ipix, int64_t n, int32_t nx,
+                                                       int32_t ny, int32_t tx, int32_t ty, int32_t* __restrict__ out,
+                                                       int64_t* nbad) {
+  const int64_t npix = (int64_t)nx * ny;
+  const int32_t nty = (ny + ty - 1) / ty;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t p = ipix[i];
+    const bool bad = p < 0 || (int64_t)p >= npix;
+    mk::count_bad(bad, nbad);
+    if (bad) {
+      out[i] = 0;  // in bounds; the caller treats any bad sample as an error
+      continue;
+    }
+    const int32_t x = p / ny, y = p - x * ny;
+    out[i] = ((x / tx) * nty + y / ty) * (tx * ty) + (x % tx) * ty + (y % ty);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) map_tiles_kernel(const T* __restrict__ map, int32_t nx, int32_t ny, int32_t tx,
+                                                        int32_t ty, T* __restrict__ tiled, int64_t ntiled) {
+  const int32_t nty = (ny + ty - 1) / ty, tp = tx * ty;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntiled; t += stride) {
+    const int32_t tile = (int32_t)(t / tp), w = (int32_t)(t - (int64_t)tile * tp);
+    const int32_t bx = tile / nty, by = tile - bx * nty;
+    const int32_t x = bx * tx + w / ty, y = by * ty + w % ty;
+    tiled[t] = (x < nx && y < ny) ? map[(int64_t)x * ny + y] : T(0);
+  }
+}
+
+int64_t tiled_size(int32_t nx, int32_t ny, int32_t tx, int32_t ty) {
+  return (int64_t)((nx + tx - 1) / tx) * tx * (int64_t)((ny + ty - 1) / ty) * ty;
+}
+
+template <typename T>
+int map_tiles_impl(const T* map, int32_t nx, int32_t ny, int32_t tx, int32_t ty, T* tiled, void* stream) {
+  MK_REQUIRE(nx > 0 && ny > 0 && tx > 0 && ty > 0 && tx * ty <= 1024, MK_ERR_ARG, "bad map or tile size");
+  MK_REQUIRE(map && tiled, MK_ERR_ARG, "null pointer");
+  const int64_t nt = tiled_size(nx, ny, tx, ty);
+  MK_REQUIRE(nt <= INT32_MAX, MK_ERR_ARG, "tiled map too large for int32 indices");
+  map_tiles_kernel<T><<<mk::grid_for(nt, 256, 256 * 16), 256, 0, mk::as_stream(stream)>>>(map, nx, ny, tx, ty,
+                                                                                           tiled, nt);
+  return mk::check_launch("map_tiles_kernel");
+}
+
 template <typename T>
 int tod2map_atomic_impl(T* map, const T* dat, int64_t nsamp, const int32_t* ipix, void* stream) {
   MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
@@ -601,4 +657,31 @@ extern "C" int mk_t2m_plan_info(const mk_t2m_plan* pl, int64_t* info) {
 extern "C" int mk_t2m_plan_destroy(mk_t2m_plan* pl) {
   free_plan(pl);
   return MK_OK;
+}
+
+extern "C" int64_t mk_map_tiled_size(int32_t nx, int32_t ny, int32_t tile_x, int32_t tile_y) {
+  if (nx <= 0 || ny <= 0 || tile_x <= 0 || tile_y <= 0) return -1;
+  return tiled_size(nx, ny, tile_x, tile_y);
+}
+
+extern "C" int mk_pix_tile_remap(const int32_t* ipix, int64_t nsamp, int32_t nx, int32_t ny, int32_t tile_x,
+                                 int32_t tile_y, int32_t* out, int64_t* dev_nbad, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  MK_REQUIRE(nx > 0 && ny > 0 && tile_x > 0 && tile_y > 0 && tile_x * tile_y <= 1024, MK_ERR_ARG,
+             "bad map or tile size");
+  MK_REQUIRE(tiled_size(nx, ny, tile_x, tile_y) <= INT32_MAX, MK_ERR_ARG, "tiled map too large for int32 indices");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(ipix && out, MK_ERR_ARG, "null pointer");
+  pix_tile_kernel<<<mk::grid_for(nsamp, 256, 256 * 32), 256, 0, mk::as_stream(stream)>>>(ipix, nsamp, nx, ny, tile_x,
+                                                                                          tile_y, out, dev_nbad);
+  return mk::check_launch("pix_tile_kernel");
+}
+
+extern "C" int mk_map_to_tiles_f32(const float* map, int32_t nx, int32_t ny, int32_t tile_x, int32_t tile_y,
+                                   float* tiled, void* stream) {
+  return map_tiles_impl<float>(map, nx, ny, tile_x, tile_y, tiled, stream);
+}
+extern "C" int mk_map_to_tiles_f64(const double* map, int32_t nx, int32_t ny, int32_t tile_x, int32_t tile_y,
+                                   double* tiled, void* stream) {
+  return map_tiles_impl<double>(map, nx, ny, tile_x, tile_y, tiled, stream);
 }

@@ -115,6 +115,10 @@ _SIGS = {
     "mk_addmul_f64": ([P, P, P, I64, P], I),
     "mk_filt_plan_create_p1": ([I, P, I, P, P], I),
     "mk_filt_hot_p1": ([P, I], I),
+    "mk_map_tiled_size": ([I32, I32, I32, I32], I64),
+    "mk_pix_tile_remap": ([P, I64, I32, I32, I32, I32, P, P, P], I),
+    "mk_map_to_tiles_f32": ([P, I32, I32, I32, I32, P, P], I),
+    "mk_map_to_tiles_f64": ([P, I32, I32, I32, I32, P, P], I),
     "mk_tod2cuts_f32": ([P, P, I64, P, I64, I, P, P], I),
     "mk_tod2cuts_f64": ([P, P, I64, P, I64, I, P, P], I),
     "mk_cuts2tod_f32": ([P, P, I64, P, I64, I, P, P], I),

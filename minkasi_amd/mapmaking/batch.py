@@ -5,7 +5,8 @@ tod2map one after another (minkasi/tods/core.py:649-691, :1012-1020).  Here
 all TODs live in one arena (concatenated row-major ndet x n blocks) and each
 stage is ONE launch over the whole arena:
 
-  1. map2tod      flat gather over the arena                 (mk_map2tod_f32)
+  1. map2tod      flat gather over the arena from a tiled copy of the map
+                  (mk_map_to_tiles_f32 + mk_map2tod_f32 with remapped indices)
   2. v.d          split-BF16 MFMA, one segment per TOD       (mk_rot_plan_exec)
   3. C W C        four-step FFT filter, one segment per TOD  (mk_filt_plan_exec_f32)
                   (edge halving folded in)
@@ -21,9 +22,15 @@ import numpy as np
 import torch
 
 from .. import device as dev
+from ..lib import mkgpu
 from ..lib.plans import FiltPlan, RotPlan, T2MPlan
 from .map2tod import map2tod
 from .noise import NoiseSmoothedSVD
+
+
+# map2tod gather tiles (tx, ty), 32 FP32 pixels = one 128-byte line; the
+# first is the default, the autotuner times all of them
+GATHER_TILES = ((8, 4), (16, 2), (4, 8), (32, 1))
 
 
 def _hot_p1(plan):
@@ -75,6 +82,22 @@ class BatchedDot:
         self.buf_b = torch.empty(max(off, 4), dtype=torch.float32, device=d)
         from .. import tune
         tuner = tune.active()
+        # gather indices into a tiled copy of the map (see _set_gather_tile)
+        self.ipix_g = torch.empty(max(off, 4), dtype=torch.int32, device=d)
+        self.map_t = None
+        self.gather_tile = None
+        if tuner is not None and off > 0:
+            sig = tune.ipix_signature(self.ipix[:off], self.nx, self.ny)
+            x = torch.randn(self.nx, self.ny, device=d)
+
+            def time_tile(t):
+                self._set_gather_tile(tuple(t))
+                return self._time(lambda: self._map2tod(x, self.buf_a[:off]))
+
+            best = tuner.choose("gather_tile", sig, [list(t) for t in GATHER_TILES], time_tile)
+            self._set_gather_tile(tuple(best))
+        else:
+            self._set_gather_tile(GATHER_TILES[0])
         if tuner is not None and off > 0 and tile == (64, 32) and item_samples is None:
             self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
         else:
@@ -93,6 +116,44 @@ class BatchedDot:
         self.graph = None
         self._gx = self._gout = None
 
+    def _set_gather_tile(self, tile):
+        """Rewrite the gather indices for tx x ty map tiles and size the
+        tiled map buffer."""
+        if tile == self.gather_tile:
+            return
+        tx, ty = int(tile[0]), int(tile[1])
+        n = self.nsamp
+        if n:
+            nbad = torch.zeros(1, dtype=torch.int64, device=self.ipix.device)
+            mkgpu.check("mk_pix_tile_remap", self.ipix.data_ptr(), n, self.nx, self.ny, tx, ty,
+                        self.ipix_g.data_ptr(), nbad.data_ptr(), dev.stream_ptr())
+            nb = int(nbad.item())
+            if nb:
+                raise ValueError("%d samples have pixel indices outside the %dx%d map" % (nb, self.nx, self.ny))
+        size = int(mkgpu.lib.mk_map_tiled_size(self.nx, self.ny, tx, ty))
+        if self.map_t is None or self.map_t.numel() != size:
+            self.map_t = torch.zeros(size, dtype=torch.float32, device=self.ipix.device)
+        self.gather_tile = (tx, ty)
+        self.graph = None
+
+    def _map2tod(self, x, a):
+        """a = x.flat[ipix] over the arena, gathered from the tiled copy."""
+        tx, ty = self.gather_tile
+        mkgpu.check("mk_map_to_tiles_f32", dev.ptr(x, torch.float32, "map"), self.nx, self.ny, tx, ty,
+                    self.map_t.data_ptr(), dev.stream_ptr())
+        map2tod(a, self.map_t, self.ipix_g[:self.nsamp], do_add=False)
+
+    @staticmethod
+    def _time(fn, reps=3):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
     def matches(self, todvec, mapset):
         m = mapset.maps[0]
         return (len(todvec.tods) == len(self.tods) and all(a is b for a, b in zip(todvec.tods, self.tods))
@@ -102,7 +163,7 @@ class BatchedDot:
     def stages(self, x, out, accumulate):
         n = self.nsamp
         a, b = self.buf_a[:n], self.buf_b[:n]
-        map2tod(a, x.reshape(-1), self.ipix[:n], do_add=False)
+        self._map2tod(x, a)
         self.rot_v.exec(a, b)
         self.filt.exec(b, b)
         self.rot_vT.exec(b, a)
@@ -158,7 +219,7 @@ class BatchedDot:
         n = self.nsamp
         a, b = self.buf_a[:n], self.buf_b[:n]
         steps = [
-            ("map2tod", lambda: map2tod(a, x.reshape(-1), self.ipix[:n], do_add=False)),
+            ("map2tod", lambda: self._map2tod(x, a)),
             ("rot_v", lambda: self.rot_v.exec(a, b)),
             ("filter", lambda: self.filt.exec(b, b)),
             ("rot_vT", lambda: self.rot_vT.exec(b, a)),

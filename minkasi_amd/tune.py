@@ -2,7 +2,7 @@
 
 It takes the role FFTW wisdom plays in the reference (minkasi/tools/fft.py:
 347-384): transform plans chosen by measurement, remembered across runs.
-Two choices are tuned on the device:
+Three choices are tuned on the device:
 
   * FFT column length of the noise filter.  The filter's real FFT has
     L = 2^11 * P1 points with P1 a 5-smooth number in [2, 256].  By default
@@ -11,6 +11,8 @@ Two choices are tuned on the device:
     point, and then picks for each row length n the P1 that minimises
     P1 * cost(P1) among the lengths that fit.  A longer transform is still
     exact, because the convolution only needs L >= 3(n-1)+1.
+  * map2tod gather tile: the batched operator gathers from a copy of the map
+    laid out in tx x ty pixel tiles; the tile shape is timed on the arena.
   * tod2map plan tile and item size.  Candidate plans are built on the
     arena's own pixelization and their exec is timed.  The winner is
     remembered for that pixelization (map size, sample count and a checksum
@@ -252,6 +254,29 @@ class Tuner:
                 print("tune: t2m tile %s items %d  %.3f ms" % (t, i, m))
         return best
 
+    # -------------------------------------------------------------- generic
+    def choose(self, kind, sig, candidates, time_fn):
+        """The fastest of candidates for (kind, sig): from the cache, or by
+        calling time_fn(candidate) -> milliseconds for each and caching the
+        winner.  Candidates are JSON-serialisable (lists are kept as lists)."""
+        table = self.entry.setdefault(kind, {})
+        if sig in table:
+            c = table[sig]
+            self.chosen[kind] = c
+            return c["choice"]
+        tried = []
+        for cand in candidates:
+            tried.append([cand, float(time_fn(cand))])
+        best = min(tried, key=lambda t: t[1])
+        c = {"choice": best[0], "ms": best[1], "tried": tried, "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+        table[sig] = c
+        self.chosen[kind] = c
+        self.save()
+        if self.verbose:
+            for cand, ms in tried:
+                print("tune: %s %s  %.3f ms" % (kind, cand, ms))
+        return best[0]
+
     def report(self):
         """What the tuner chose, for the benchmark line."""
         out = {"cache": self.path, "device": self.key}
@@ -259,4 +284,6 @@ class Tuner:
             out["t2m_tile"] = self.chosen["t2m"]["tile"]
             out["t2m_item_samples"] = self.chosen["t2m"]["item_samples"]
         out["fft_p1_changed"] = self.chosen.get("fft_p1_changed", 0)
+        if "gather_tile" in self.chosen:
+            out["gather_tile"] = self.chosen["gather_tile"]["choice"]
         return out

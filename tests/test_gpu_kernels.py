@@ -280,3 +280,71 @@ def test_pcg_vector_ops(env):
         mkgpu.check("mk_dot_f32", xt.data_ptr(), rt.data_ptr(), n, sc.data_ptr(), S.work.data_ptr(), sp)
         vals.append(sc[mkgpu.SC_DOT].item())
     assert vals[0] == vals[1] == vals[2]
+
+
+def _tiled_index_np(ipix, ny, tx, ty):
+    x, y = ipix // ny, ipix % ny
+    nty = -(-ny // ty)
+    return ((x // tx) * nty + y // ty) * (tx * ty) + (x % tx) * ty + y % ty
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tiled_gather_layout_is_bit_exact(env, dtype):
+    """mk_pix_tile_remap and mk_map_to_tiles_* against the layout formula;
+    map2tod from the tiled copy equals map.ravel()[ipix] bit for bit, for
+    ragged map sizes and every tile shape the batched operator uses."""
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(8)
+    sfx = "f64" if dtype == np.float64 else "f32"
+    for nx, ny in ((37, 53), (64, 64), (1, 5), (300, 7)):
+        m = rng.standard_normal((nx, ny)).astype(dtype)
+        ipix = rng.integers(0, nx * ny, size=4 * 1001).astype(np.int32)
+        ipix[:3] = [0, nx * ny - 1, ny - 1]
+        d_m, d_ip = device.to_device(m), device.to_device(ipix)
+        for tx, ty in ((8, 4), (16, 2), (4, 8), (32, 1), (3, 5)):
+            size = int(mkgpu.lib.mk_map_tiled_size(nx, ny, tx, ty))
+            assert size == (-(-nx // tx) * tx) * (-(-ny // ty) * ty)
+            tiled = torch.full((size,), 7.0, dtype=d_m.dtype, device=d_m.device)
+            mkgpu.check("mk_map_to_tiles_" + sfx, d_m.data_ptr(), nx, ny, tx, ty, tiled.data_ptr(),
+                        device.stream_ptr())
+            want = np.zeros(size, dtype)
+            xs, ys = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+            want[_tiled_index_np((xs * ny + ys).ravel(), ny, tx, ty)] = m.ravel()
+            assert np.array_equal(tiled.cpu().numpy(), want), (nx, ny, tx, ty)
+            it = torch.empty_like(d_ip)
+            nbad = torch.zeros(1, dtype=torch.int64, device=d_ip.device)
+            mkgpu.check("mk_pix_tile_remap", d_ip.data_ptr(), ipix.size, nx, ny, tx, ty, it.data_ptr(),
+                        nbad.data_ptr(), device.stream_ptr())
+            assert int(nbad.item()) == 0
+            assert np.array_equal(it.cpu().numpy(), _tiled_index_np(ipix.astype(np.int64), ny, tx, ty))
+            out = torch.empty(ipix.size, dtype=d_m.dtype, device=d_m.device)
+            mkgpu.check("mk_map2tod_" + sfx, out.data_ptr(), tiled.data_ptr(), ipix.size, it.data_ptr(), 0,
+                        device.stream_ptr())
+            assert np.array_equal(out.cpu().numpy(), m.ravel()[ipix])
+    # indices outside the map are counted (and written as 0)
+    bad = np.array([0, -1, 12, 12 * 13, 5], dtype=np.int32)
+    d_b = device.to_device(bad)
+    it = torch.empty_like(d_b)
+    nbad = torch.zeros(1, dtype=torch.int64, device=d_b.device)
+    mkgpu.check("mk_pix_tile_remap", d_b.data_ptr(), 5, 12, 13, 8, 4, it.data_ptr(), nbad.data_ptr(),
+                device.stream_ptr())
+    assert int(nbad.item()) == 2
+    assert it.cpu().numpy()[1] == 0 and it.cpu().numpy()[3] == 0
+
+
+def test_batched_gather_equals_rowmajor_gather(env):
+    """The batched operator's map2tod stage (tiled copy, remapped indices)
+    reproduces map.ravel()[ipix] exactly for every gather tile."""
+    torch, device, mkgpu = env
+    from minkasi_amd.mapmaking.batch import GATHER_TILES
+    from minkasi_amd.workloads import build_problem
+    P = build_problem(1, ntod=2, verbose=False)
+    op = P.op
+    n = op.nsamp
+    x = torch.randn(op.nx, op.ny, device=op.buf_a.device)
+    want = x.reshape(-1)[op.ipix[:n].long()]
+    for t in GATHER_TILES:
+        op._set_gather_tile(t)
+        a = torch.empty(n, dtype=torch.float32, device=x.device)
+        op._map2tod(x, a)
+        assert torch.equal(a, want), t
