@@ -19,7 +19,7 @@ Three choices are tuned on the device:
     of a strided sample of ipix).
 
 The cache is a JSON file at $MK_TUNE_CACHE or ~/.cache/minkasi_amd/tune.json,
-keyed by device name and library ABI version.  Tuning is opt-in:
+keyed by GPU architecture, CU count and library ABI version.  Tuning is opt-in:
 tune.enable() (bench.py does this unless --no-tune) makes later FiltPlans
 and BatchedDots use it.  Plans built before it keep their default choices,
 and BatchedDot copies each TOD's column length from that TOD's noise model.
@@ -124,9 +124,15 @@ class Tuner:
     # -------------------------------------------------------------- keys, cache
     @staticmethod
     def _device_key():
+        """Architecture, CU count and library ABI (the marketing name is not
+        used: it can be empty, e.g. under a profiler)."""
         import torch
         from .lib import mkgpu
-        name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+        if torch.cuda.is_available():
+            pr = torch.cuda.get_device_properties(0)
+            name = "%s|cu%d" % (getattr(pr, "gcnArchName", "") or pr.name, pr.multi_processor_count)
+        else:
+            name = "cpu"
         return "%s|abi%d" % (name, int(mkgpu.lib.mk_version()))
 
     @staticmethod
