@@ -133,8 +133,11 @@ def _plan(env, ipix_t, nx, ny, tx=64, ty=32, item=4096):
     return h
 
 
-@pytest.mark.parametrize("dtype", ["float32", "float64"])
-def test_tod2map_plan_matches_oracle_and_is_deterministic(env, dtype):
+@pytest.mark.parametrize("dtype,tile", [("float32", (64, 32)), ("float64", (64, 32)), ("float32", (32, 32)),
+                                        ("float64", (32, 32))])
+def test_tod2map_plan_matches_oracle_and_is_deterministic(env, dtype, tile):
+    """Two tile shapes: the default 64 x 32 and the 32 x 32 the autotuner picks
+    on the benchmark workload."""
     torch, device, mkgpu = env
     rng = np.random.default_rng(4)
     ra_b, dec_b, dra, ddec = raster_pointing(rng, 64, 65536)
@@ -148,7 +151,7 @@ def test_tod2map_plan_matches_oracle_and_is_deterministic(env, dtype):
     ref = np.zeros(nx * ny)
     core.tod2map(ref, tod, ipix)
     ipt = device.to_device(ipix)
-    h = _plan(env, ipt, nx, ny)
+    h = _plan(env, ipt, nx, ny, *tile)
     try:
         td = device.to_device(tod.astype(dtype))
         name = "mk_t2m_plan_exec_f32" if dtype == "float32" else "mk_t2m_plan_exec_f64"
@@ -186,7 +189,7 @@ def test_tod2map_plan_edge_cases(env):
     ref = np.bincount(ipix.ravel(), tod.ravel(), minlength=nx * ny)
     ipt = device.to_device(ipix)
     tdev = device.to_device(tod)
-    for tx, ty, item in [(8, 8, 64), (16, 4, 128), (64, 32, 4096)]:
+    for tx, ty, item in [(8, 8, 64), (16, 4, 128), (64, 32, 4096), (32, 32, 192), (4, 4, 64)]:
         h = _plan(env, ipt, nx, ny, tx, ty, item)
         try:
             mt = torch.zeros(nx * ny, dtype=torch.float64, device=ipt.device)
