@@ -236,3 +236,123 @@ class BatchedDot:
             e1.synchronize()
             res[name] = e0.elapsed_time(e1) / reps
         return res
+
+
+class BatchedCutsDot:
+    """The batched operator for a Mapset of one FP32 SkyMap plus a tsModel of
+    CutsCompact (the reference's cuts example, examples/minkasi_cuts_example.py),
+    on flat vectors [map (nx*ny) | every TOD's cut values, in TOD order].
+
+    A [x; c] = [P^T N^-1 (P x + C c); C^T N^-1 (P x + C c)] with C the cut
+    scatter (mk_cuts2tod_f32 over one arena-wide int64 index, after the
+    map2tod stage) and C^T the gather (mk_tod2cuts_f32, after the second
+    rotation), so every stage stays one launch over all TODs."""
+
+    @staticmethod
+    def eligible(todvec, mapset):
+        from ..tods.cuts import CutsCompact
+        from .timestream import tsModel, tod_key
+        if len(mapset.maps) != 2 or not isinstance(mapset.maps[1], tsModel):
+            return False
+        one = type(mapset)()
+        one.maps, one.nmap = [mapset.maps[0]], 1
+        if not BatchedDot.eligible(todvec, one):
+            return False
+        cm = mapset.maps[1]
+        for tod in todvec.tods:
+            c = cm.data.get(tod_key(tod))
+            if not isinstance(c, CutsCompact) or c.map.dtype != torch.float32 or not c._unique:
+                return False
+            if c.ndet * c.imax != tod.get_nsamp():
+                return False
+        return len(cm.data) == todvec.ntod
+
+    def __init__(self, base, cuts_model):
+        from .timestream import tod_key
+        self.base = base
+        self.keys = [tod_key(t) for t in base.tods]
+        self.npix = base.nx * base.ny
+        parts, self.counts = [], []
+        for key, o in zip(self.keys, base.offs):
+            c = cuts_model.data[key]
+            parts.append(c.imap + o)
+            self.counts.append(int(c.imap.numel()))
+        d = base.buf_a.device
+        self.cimap = torch.cat(parts) if parts else torch.zeros(0, dtype=torch.int64, device=d)
+        self.ncut = int(self.cimap.numel())
+        self.n = self.npix + self.ncut
+        self.cut_ids = [id(cuts_model.data[k].imap) for k in self.keys]
+
+    def matches(self, todvec, mapset):
+        """Same TODs, map size and cut samples (a Mapset copy holds copies of
+        the cut indices: those are compared by value)."""
+        if len(mapset.maps) != 2 or not hasattr(mapset.maps[1], "data") or not self.base.matches(todvec, mapset):
+            return False
+        cm = mapset.maps[1]
+        for j, k in enumerate(self.keys):
+            c = cm.data.get(k)
+            if c is None or int(c.imap.numel()) != self.counts[j]:
+                return False
+            if id(c.imap) != self.cut_ids[j] and not torch.equal(c.imap, self._imap_of(j)):
+                return False
+        return True
+
+    def _imap_of(self, j):
+        o = sum(self.counts[:j])
+        return self.cimap[o:o + self.counts[j]] - self.base.offs[j]
+
+    # ---------------------------------------------------------------- packing
+    def pack(self, mapset, out=None):
+        """Flat FP32 vector [map | cuts] of a Mapset of this structure."""
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.float32, device=self.base.buf_a.device)
+        out[:self.npix].copy_(mapset.maps[0].map.reshape(-1))
+        o = self.npix
+        for k, c in zip(self.keys, self.counts):
+            if c:
+                out[o:o + c].copy_(mapset.maps[1].data[k].map)
+            o += c
+        return out
+
+    def unpack(self, flat, mapset):
+        mapset.maps[0].map.copy_(flat[:self.npix].view_as(mapset.maps[0].map))
+        o = self.npix
+        for k, c in zip(self.keys, self.counts):
+            if c:
+                mapset.maps[1].data[k].map.copy_(flat[o:o + c])
+            o += c
+        return mapset
+
+    # ---------------------------------------------------------------- operator
+    def _cuts_in(self, c, a):
+        if self.ncut:
+            mkgpu.check("mk_cuts2tod_f32", a.data_ptr(), c.data_ptr(), a.numel(), self.cimap.data_ptr(), self.ncut, 1,
+                        None, dev.stream_ptr())
+
+    def _cuts_out(self, a, c, accumulate):
+        if self.ncut:
+            mkgpu.check("mk_tod2cuts_f32", c.data_ptr(), a.data_ptr(), a.numel(), self.cimap.data_ptr(), self.ncut,
+                        int(bool(accumulate)), None, dev.stream_ptr())
+
+    def apply(self, x, out, accumulate=True):
+        """out (+)= A x on flat vectors."""
+        B = self.base
+        n = B.nsamp
+        a, b = B.buf_a[:n], B.buf_b[:n]
+        xm = x[:self.npix].view(B.nx, B.ny)
+        B._map2tod(xm, a)
+        self._cuts_in(x[self.npix:], a)
+        B.rot_v.exec(a, b)
+        B.filt.exec(b, b)
+        B.rot_vT.exec(b, a)
+        B.t2m.exec(out[:self.npix], a, accumulate)
+        self._cuts_out(a, out[self.npix:], accumulate)
+        return out
+
+    def rhs(self, out, accumulate=True):
+        """out (+)= [P^T N^-1 d; C^T N^-1 d] (TodVec.make_rhs)."""
+        B = self.base
+        n = B.nsamp
+        B.rhs(out[:self.npix].view(B.nx, B.ny), accumulate)  # leaves N^-1 d in buf_a
+        self._cuts_out(B.buf_a[:n], out[self.npix:], accumulate)
+        return out

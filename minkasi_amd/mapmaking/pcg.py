@@ -66,6 +66,16 @@ def run_pcg_wprior(b, x0, tods, prior=None, precon=None, mask=None, maxiter=25, 
     if fast:
         return _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
                                true_residual_every, history, verbose)
+    bc = None
+    if prior is None and mask is None and hasattr(tods, "batched_cuts"):
+        bc = tods.batched_cuts(b)
+        if bc is not None and not (bc.matches(tods, x0) and (precon is None or (
+                hasattr(precon, "maps") and len(precon.maps) == 2 and bc.matches(tods, precon)
+                and precon.maps[0].map.dtype == torch.float32))):
+            bc = None
+    if bc is not None:
+        return _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
+                               true_residual_every, history, verbose, flat=bc)
     return _run_pcg_generic(b, x0, tods, prior, precon, mask, maxiter, outroot, save_iters, save_ind, save_tail,
                             tol, history, verbose)
 
@@ -157,8 +167,10 @@ class DevicePCG:
         self.z = torch.empty_like(b_map)
         self.p = torch.empty_like(b_map)
         self.Ap = torch.empty_like(b_map)
-        self.jacobi = precon is not None and _single_f32_map(precon)
-        self.w = precon.maps[0].map if self.jacobi else None
+        # Jacobi: a Mapset holding the weight map, or the weights as a tensor
+        # shaped like b (the flat [map | cuts] vectors of BatchedCutsDot)
+        self.jacobi = precon is not None and (isinstance(precon, torch.Tensor) or _single_f32_map(precon))
+        self.w = (precon if isinstance(precon, torch.Tensor) else precon.maps[0].map) if self.jacobi else None
         self.precon = precon
         self.x = None
         self.bb = None
@@ -243,12 +255,24 @@ class DevicePCG:
 
 
 def _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind, save_tail, tol,
-                    true_residual_every, history, verbose):
-    """Device-scalar PCG with the same update order as pcg.py:143-170."""
-    op = tods.batched(b)
+                    true_residual_every, history, verbose, flat=None):
+    """Device-scalar PCG with the same update order as pcg.py:143-170.
+    flat: a BatchedCutsDot, for a Mapset of a map and cut samples solved as
+    one flat vector (packed here, unpacked into the returned Mapset)."""
     x = x0.copy()
-    solver = DevicePCG(op, b.maps[0].map, precon)
-    solver.init(x.maps[0].map)
+    if flat is None:
+        op = tods.batched(b)
+        solver = DevicePCG(op, b.maps[0].map, precon)
+        solver.init(x.maps[0].map)
+    else:
+        w = flat.pack(precon) if precon is not None else None
+        null = None
+        if w is not None:
+            # the null mode of A is the constant on hit pixels, zero on the cuts
+            null = torch.zeros(flat.n, dtype=torch.bool, device=w.device)
+            null[:flat.npix] = w[:flat.npix] > 0
+        solver = DevicePCG(flat, flat.pack(b), w, null_mask=null)
+        solver.init(flat.pack(x))
     nsamp = tods.get_nsamp()
     tloop = time.time()
     it_done = 0
@@ -272,7 +296,11 @@ def _run_pcg_device(b, x0, tods, precon, maxiter, outroot, save_iters, save_ind,
             if tol is not None and rel < tol:
                 break
         if i in save_iters and myrank == 0:
+            if flat is not None:
+                flat.unpack(solver.x, x)
             x.maps[save_ind].write(outroot + "_" + str(i) + save_tail)
+    if flat is not None:
+        flat.unpack(solver.x, x)
     torch.cuda.synchronize()
     if verbose and it_done:
         tave = (time.time() - tloop) / it_done

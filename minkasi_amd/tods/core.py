@@ -232,6 +232,7 @@ class TodVec:
         self.tods[-1].set_tag(self.ntod)
         self.ntod = self.ntod + 1
         self._batch = None
+        self._batch_cuts = None
 
     def lims(self):
         if self.ntod == 0:
@@ -268,6 +269,20 @@ class TodVec:
             self._batch = b
         return b
 
+    def batched_cuts(self, mapset):
+        """The batched operator for a Mapset of one FP32 SkyMap plus a tsModel
+        of CutsCompact (built once per cut layout, cached), or None."""
+        from ..mapmaking.batch import BatchedCutsDot
+        if not BatchedCutsDot.eligible(self, mapset):
+            return None
+        bc = getattr(self, "_batch_cuts", None)
+        if bc is None or not bc.matches(self, mapset):
+            one = type(mapset)()
+            one.maps, one.nmap = [mapset.maps[0]], 1
+            bc = BatchedCutsDot(self.batched(one), mapset.maps[1])
+            self._batch_cuts = bc
+        return bc
+
     def dot_cached(self, mapset, mapset_out=None, mask=None):
         return self.dot(mapset, mapset_out, mask=mask)
 
@@ -283,6 +298,12 @@ class TodVec:
             b.apply(mapset.maps[0].map, mapset_out.maps[0].map, accumulate=True)
             if mask is not None:
                 mapset_out.maps[0].map *= mask.maps[0].map
+            return mapset_out
+        bc = None if (report_times or mask is not None) else self.batched_cuts(mapset)
+        if bc is not None:
+            y = bc.pack(mapset_out)
+            bc.apply(bc.pack(mapset), y, accumulate=True)
+            bc.unpack(y, mapset_out)
             return mapset_out
         times = np.zeros(self.ntod)
         tot_times = np.zeros(3)
@@ -306,6 +327,12 @@ class TodVec:
         b = self.batched(mapset)
         if b is not None:
             b.rhs(mapset.maps[0].map, accumulate=True)
+            return
+        bc = self.batched_cuts(mapset)
+        if bc is not None:
+            y = bc.pack(mapset)
+            bc.rhs(y, accumulate=True)
+            bc.unpack(y, mapset)
             return
         for tod in self.tods:
             dat_filt = tod.apply_noise()

@@ -414,3 +414,56 @@ def test_polmap_pcg_matches_oracle(env):
     M = lambda r: oc.precon_mul(pre_ref, r.reshape(pm.nx, pm.ny, 3), "IQU_PRECON").ravel()
     xr, _ = solve.run_pcg(b, np.zeros_like(b), A, M, maxiter=10)
     assert rel_err(xs.maps[0].map.cpu().numpy().ravel(), xr) < 1e-8
+
+
+def test_batched_map_plus_cuts_matches_per_tod_path(env):
+    """FP32 SkyMap + tsModel(CutsCompact): the batched flat operator
+    (BatchedCutsDot) gives the per-TOD loop's A x, rhs and PCG iterates."""
+    torch, device, mkgpu = env
+    import minkasi_amd as mk
+    from minkasi_amd.mapmaking import pcg
+    from minkasi_amd.mapmaking.timestream import tod_key
+    from tests.test_gpu_api import build
+    todvec, m, otods = build(ntod=3, ndet=32, n=20001, dtype=np.float32, seed=5)
+    rng = np.random.default_rng(11)
+    cuts = mk.tsModel(todvec, mk.CutsCompact)
+    for tod in todvec.tods:
+        mask = np.ones(tod.get_data_dims(), bool)
+        for _ in range(20):  # segments of 50-400 samples on random detectors
+            d, t0 = rng.integers(0, mask.shape[0]), rng.integers(0, mask.shape[1] - 400)
+            mask[d, t0:t0 + rng.integers(50, 400)] = False
+        cuts.data[tod_key(tod)].cuts_from_array(mask)
+        cuts.data[tod_key(tod)].get_imap()
+    ms = mk.Mapset()
+    ms.add_map(m)
+    ms.add_map(cuts)
+    bc = todvec.batched_cuts(ms)
+    assert bc is not None and bc.ncut > 0
+    x = ms.copy()
+    x.maps[0].assign(rng.standard_normal((m.nx, m.ny)).astype(np.float32))
+    for k in x.maps[1].data:
+        c = x.maps[1].data[k]
+        c.map.copy_(torch.as_tensor(rng.standard_normal(c.map.numel()), device=c.map.device))
+    fast = todvec.dot(x)
+    slow, _ = todvec.dot(x, report_times=True)  # the reference's per-TOD loop
+    pack = lambda s: bc.pack(s).cpu().numpy()
+    assert rel_err(pack(fast), pack(slow)) < 1e-5
+    rf = ms.copy()
+    todvec.make_rhs(rf)
+    rs = ms.copy()
+    for tod in todvec.tods:
+        dat_filt = tod.apply_noise()
+        for mm in rs.maps:
+            mm.tod2map(tod, dat_filt)
+    assert rel_err(pack(rf), pack(rs)) < 1e-5
+    hits = mk.make_hits(todvec, m)
+    pre = ms.copy()
+    h = hits.map.cpu().numpy()
+    pre.maps[0].assign(np.where(h > 0, 1.0 / np.where(h > 0, h, 1), 0.0).astype(np.float32))
+    for k in pre.maps[1].data:
+        pre.maps[1].data[k].map.fill_(1.0)
+    x0 = rf.copy()
+    x0.clear()
+    xf = mk.run_pcg(rf, x0, todvec, pre, maxiter=10, verbose=False)
+    xs = pcg._run_pcg_generic(rf, x0, todvec, None, pre, None, 10, "map", [-1], 0, ".fits", None, None, False)
+    assert rel_err(pack(xf), pack(xs)) < 1e-3
