@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cuts", type=float, default=0.0, metavar="FRACTION",
+                    help="also solve for cut samples: this fraction of the samples, in 200-sample segments on random "
+                         "detectors, as a tsModel of CutsCompact next to the map (one flat PCG vector)")
     ap.add_argument("--no-tune", action="store_true",
                     help="default plans instead of the autotuner's (FFT column lengths, tod2map tiles)")
     ap.add_argument("--truncate", action="store_true",
@@ -141,6 +144,38 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": int(n * 4), "launches": 3}))
 
 
+def add_cuts(P, frac, seg=200, seed=1234):
+    """Cut model for the cuts variant: on every TOD, segments of `seg` samples
+    on random detectors covering about `frac` of its samples.  Returns the
+    flat operator, rhs, Jacobi weights [1/hits | 1], null mask and cut count."""
+    import contextlib
+    import numpy as np
+    import torch
+    from minkasi_amd.mapmaking.batch import BatchedCutsDot
+    from minkasi_amd.mapmaking.timestream import tod_key, tsModel
+    from minkasi_amd.tods.cuts import CutsCompact
+    rng = np.random.default_rng(seed)
+    cm = tsModel(P.todvec, CutsCompact)
+    with contextlib.redirect_stdout(sys.stderr):  # get_imap prints the cut count, as the reference does
+        for tod in P.todvec.tods:
+            c = cm.data[tod_key(tod)]
+            ndet, n = tod.get_data_dims()
+            for _ in range(max(1, int(frac * ndet * n / seg))):
+                d, t0 = int(rng.integers(ndet)), int(rng.integers(0, max(1, n - seg)))
+                c.add_cut(d, t0, t0 + seg)
+            c.merge_cuts()
+            c.get_imap()
+    op = BatchedCutsDot(P.op, cm)
+    b = torch.zeros(op.n, dtype=torch.float32, device=P.op.buf_a.device)
+    op.rhs(b, accumulate=False)
+    npix = op.npix
+    w = torch.ones(op.n, dtype=torch.float32, device=b.device)
+    w[:npix] = P.precon.maps[0].map.reshape(-1)
+    null = torch.zeros(op.n, dtype=torch.bool, device=b.device)
+    null[:npix] = w[:npix] > 0
+    return op, b, w, null, op.ncut
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -170,8 +205,13 @@ def main():
         P.t_generate, P.t_pixelize, P.t_noise, P.t_plans, P.t_rhs, len(P.shapes), P.nsamp))
     op = P.op
     b = P.rhs.maps[0].map
+    ncut = 0
+    if args.cuts > 0:
+        op, b, w, null, ncut = add_cuts(P, args.cuts)
+        solver = DevicePCG(op, b, w, null_mask=null)
+    else:
+        solver = DevicePCG(op, b, P.precon)
     # one graph for A.p on the solver's own buffers
-    solver = DevicePCG(op, b, P.precon)
     x = torch.zeros_like(b)
     solver.init(x)
     if not args.no_graph:
@@ -194,7 +234,8 @@ def main():
         ms_per_step, iters_per_s, P.nsamp * iters_per_s / 1e6, args.warmup + args.steps, rel_after))
 
     # ---- per-stage times (events on the launch stream) and roofline
-    st = op.stage_times(solver.p, torch.empty_like(b), reps=3)
+    pm = solver.p[:P.map.nx * P.map.ny].view(P.map.nx, P.map.ny)
+    st = P.op.stage_times(pm, torch.empty_like(pm), reps=3)
     sb = stage_bytes(P)
     dom = max(st, key=st.get)
     achieved = sb[dom] / (st[dom] * 1e-3) / 1e9
@@ -213,7 +254,9 @@ def main():
                                "Jacobi PCG%s" % (args.config, len(P.shapes), CONFIGS[args.config]["ndet"],
                                                  CONFIGS[args.config]["n"], P.map.nx, P.map.ny,
                                                  CONFIGS[args.config]["pix"],
-                                                 ", truncate_tod lengths" if args.truncate else ""),
+                                                 (", truncate_tod lengths" if args.truncate else "") +
+                                                 (", %d cut samples (%.2g%%) solved with the map" %
+                                                  (ncut, 100.0 * ncut / P.nsamp) if ncut else "")),
                    "ntod": len(P.shapes), "nsamp": int(P.nsamp), "map": [P.map.nx, P.map.ny]},
         "msamp_per_s": round(P.nsamp * iters_per_s / 1e6, 1),
         "setup_s": round(t_setup, 1),
@@ -226,7 +269,7 @@ def main():
         return
 
     # ---- time to tolerance (recursive residual): Jacobi, then the learned preconditioner
-    if not args.no_solve:
+    if not args.no_solve and not ncut:  # the solve-to-tolerance runs are of the map-only problem
         from minkasi_amd.workloads import map_distance, solve_to_tol
         res = {}
         jac = solve_to_tol(P, None, args.tol, args.maxiter)
@@ -248,7 +291,7 @@ def main():
         elif not args.no_learned:
             res["learned"] = {"error": "weights file %s missing" % args.weights}
         result["time_to_tol"] = res
-    if not args.no_cpu:
+    if not args.no_cpu and not ncut:
         try:
             result["cpu_baseline"] = cpu_baseline(P)
         except Exception as e:  # reported, not fatal

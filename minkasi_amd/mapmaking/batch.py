@@ -335,7 +335,29 @@ class BatchedCutsDot:
                         int(bool(accumulate)), None, dev.stream_ptr())
 
     def apply(self, x, out, accumulate=True):
-        """out (+)= A x on flat vectors."""
+        """out (+)= A x on flat vectors (a captured graph is replayed for the
+        tensors it was captured with)."""
+        g = getattr(self, "graph", None)
+        if g is not None and x is self._gx and out is self._gout and accumulate == self._gacc:
+            g.replay()
+            return out
+        return self.stages(x, out, accumulate)
+
+    def capture(self, x, out, accumulate=False, warmup=1):
+        """Captures apply(x, out) into a HIP graph (as BatchedDot.capture)."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self.stages(x, out, accumulate)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.stages(x, out, accumulate)
+        self.graph, self._gx, self._gout, self._gacc = g, x, out, accumulate
+        return g
+
+    def stages(self, x, out, accumulate):
         B = self.base
         n = B.nsamp
         a, b = B.buf_a[:n], B.buf_b[:n]
