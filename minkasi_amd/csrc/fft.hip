@@ -941,13 +941,32 @@ __global__ void __launch_bounds__(kThreads, 2) k1_hot_kernel(const Seg* __restri
   ct_fft_cm<false, P1, CW>(buf, st);
   cx<float>* trow = tbuf + sg.t_off + (int64_t)row * sg.t_stride;
   constexpr int NO = (NPTS + kThreads - 1) / kThreads;
+  // The thread's column c is the same for every u (kThreads is a multiple of
+  // CW) and k1 grows by kThreads / CW per step, so the twiddles w^(2 (j2_0+c) k1)
+  // form a geometric sequence: one complex multiply per step, re-seeded from
+  // the exact tables every 4 steps so FP32 rounding does not build up.
+  if constexpr (kThreads % CW == 0) {
+    const int c = threadIdx.x & (CW - 1), j2 = j2_0 + c;
+    const cx<float> step = tw2P(F, Ct, 2 * ((j2 * (kThreads / CW)) % P));
+    cx<float> tw{1.f, 0.f};
 #pragma unroll
-  for (int u = 0; u < NO; u++) {
-    const int idx = threadIdx.x + u * kThreads;
-    if (NO * kThreads != NPTS && idx >= NPTS) break;
-    const int k1 = idx >> LCW, c = idx & (CW - 1);
-    const int a = ((j2_0 + c) * k1) % P;  // < 2^11 * 2^8
-    trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[idx], tw2P(F, Ct, 2 * a));
+    for (int u = 0; u < NO; u++) {
+      const int idx = threadIdx.x + u * kThreads;
+      if (NO * kThreads != NPTS && idx >= NPTS) break;
+      const int k1 = idx >> LCW;
+      if ((u & 3) == 0) tw = tw2P(F, Ct, 2 * ((j2 * k1) % P));  // < 2^11 * 2^8
+      trow[k1 * kHotP2 + j2] = cmul(buf[idx], tw);
+      tw = cmul(tw, step);
+    }
+  } else {  // CW > kThreads (short columns): the column changes with u
+#pragma unroll
+    for (int u = 0; u < NO; u++) {
+      const int idx = threadIdx.x + u * kThreads;
+      if (NO * kThreads != NPTS && idx >= NPTS) break;
+      const int k1 = idx >> LCW, c = idx & (CW - 1);
+      const int a = ((j2_0 + c) * k1) % P;  // < 2^11 * 2^8
+      trow[k1 * kHotP2 + j2_0 + c] = cmul(buf[idx], tw2P(F, Ct, 2 * a));
+    }
   }
 }
 
