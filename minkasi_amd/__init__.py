@@ -39,6 +39,9 @@ def __getattr__(name):
     if name in ("tsModel", "tsMultiModel"):
         from .mapmaking import timestream
         return getattr(timestream, name)
+    if name in ("myrank", "nproc", "comm", "have_mpi", "MPI", "barrier", "report_mpi", "set_nthread", "get_nthread"):
+        from . import parallel
+        return getattr(parallel, name)
     if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens"):
         from .tods import processing
         return getattr(processing, name)

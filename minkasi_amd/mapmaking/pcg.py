@@ -20,13 +20,12 @@ import torch
 
 from .. import device as dev
 from ..lib import mkgpu
+from ..parallel import myrank
 
 try:
     from typing import Protocol, runtime_checkable
 except ImportError:  # pragma: no cover
     from typing_extensions import Protocol, runtime_checkable
-
-myrank = 0
 
 
 @runtime_checkable

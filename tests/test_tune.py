@@ -72,3 +72,17 @@ def test_cache_round_trip(tmp_path, monkeypatch):
     assert t2.entry["t2m"]["64x64:100:abc"]["tile"] == [32, 64]
     monkeypatch.setenv("MK_TUNE_CACHE", path)
     assert tune.cache_path() == path
+
+
+def test_flat_namespace_and_single_rank_names():
+    """minkasi_amd.minkasi_all gives the names legacy scripts import, and the
+    MPI bootstrap names describe one process (rank 0 of 1)."""
+    import minkasi_amd as mk
+    import minkasi_amd.minkasi_all as flat
+    for name in ("Tod", "TodVec", "SkyMap", "PolMap", "Mapset", "NoiseSmoothedSVD", "run_pcg", "run_pcg_wprior",
+                 "make_hits", "tsModel", "CutsCompact", "truncate_tod", "downsample_tod", "fit_cm_plus_poly",
+                 "fft_r2r", "myrank", "nproc", "barrier", "comm"):
+        assert hasattr(flat, name), name
+    assert (mk.myrank, mk.nproc, mk.have_mpi, mk.comm) == (0, 1, False, None)
+    assert mk.get_nthread() >= 1 and mk.set_nthread(4) is None
+    mk.barrier()  # no device work queued, no GPU needed
