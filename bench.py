@@ -181,9 +181,12 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    # stdout carries the one JSON result line; everything else (library
+    # progress prints included) goes to stderr
+    json_out, sys.stdout = sys.stdout, sys.stderr
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the batched solver holds every TOD on one device "
-              "(multi-GPU sharding over TODs is described in DESIGN.md but not built).", file=sys.stderr)
+              "(sharding the TODs over several GPUs is designed but not built).", file=sys.stderr)
         sys.exit(2)
     import torch
     from minkasi_amd.mapmaking.pcg import DevicePCG
@@ -265,7 +268,7 @@ def main():
     if tuner is not None:
         result["config"]["tuning"] = dict(tuner.report(), fft_measure_s=round(t_tune, 1))
     if args.profile:
-        print(json.dumps(result))
+        print(json.dumps(result), file=json_out, flush=True)
         return
 
     # ---- time to tolerance (recursive residual): Jacobi, then the learned preconditioner
@@ -296,7 +299,7 @@ def main():
             result["cpu_baseline"] = cpu_baseline(P)
         except Exception as e:  # reported, not fatal
             result["cpu_baseline"] = {"error": repr(e)}
-    print(json.dumps(result))
+    print(json.dumps(result), file=json_out, flush=True)
 
 
 if __name__ == "__main__":

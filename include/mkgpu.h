@@ -288,6 +288,24 @@ int mk_rot_plan_create(int nseg, const int64_t *desc, int mode,
 int mk_rot_plan_exec(const mk_rot_plan *plan, const float *b_base,
                      float *c_base, void *stream);
 int mk_rot_plan_destroy(mk_rot_plan *plan);
+/* Deflation of dominant detector modes ahead of the FP32 rotation (no
+ * counterpart in the reference, which rotates in FP64: noise.py:704).  For
+ * each segment (a TOD of ndet rows of n samples at element offset off) and
+ * its leading k <= 8 eigenvectors U (ndet x k, the first k rows of v
+ * transposed): exec stores c = U^T x (FP64 sums) in the plan and writes
+ * out = x - U c (out may equal x); addback adds c to the first k rows of the
+ * rotated y = v.out, which gives v.x exactly in exact arithmetic and keeps
+ * the FP32 rotation free of the cancellation a strong common mode causes.
+ * desc (host): 4 int64 per segment: ndet, n, off, k.  u_host: every
+ * segment's U in order, FP64 row-major.  exec and addback must pair up (c
+ * lives in the plan). */
+typedef struct mk_defl_plan mk_defl_plan;
+int mk_defl_plan_create(int nseg, const int64_t *desc, const double *u_host,
+                        mk_defl_plan **plan);
+int mk_defl_plan_exec(const mk_defl_plan *plan, const float *in, float *out,
+                      void *stream);
+int mk_defl_plan_addback(const mk_defl_plan *plan, float *y, void *stream);
+int mk_defl_plan_destroy(mk_defl_plan *plan);
 /* FP64 setup GEMMs (NoiseSmoothedSVD.__init__, noise.py:650 and :667):
  * C = A.B (A FP64; B FP32 if b_is_f32 else FP64), and the Gram matrix
  * G = D.D^T (M x M) that replaces the SVD's left singular vectors

@@ -484,3 +484,189 @@ extern "C" int mk_gram_f64(int64_t M, int64_t n, const void* D, int d_is_f32, in
   sum_partials_f64<<<mk::grid_for(M * M, 256), 256, 0, st>>>((const double*)work, nchunk, M * M, G);
   return mk::check_launch("sum_partials_f64");
 }
+
+// ---------------------------------------------------------------------------
+// Deflation of dominant detector modes ahead of the FP32 rotation.
+//
+// The rows of v are eigenvectors of the detector covariance.  When a few
+// modes carry most of the power (a strong common mode), the small rows of
+// y = v.d come out of heavy cancellation, and FP32 accumulation leaves them
+// with a relative error of about 1e-8 times the ratio of the largest row
+// norm to theirs.  With U the leading k eigenvectors (the first k rows of v,
+// transposed), c = U^T d (summed in FP64) and d_rest = d - U c,
+//   v.d = v.d_rest + [c; 0]   (v U = [I; 0]),
+// so the rotation sees d_rest, which holds no dominant mode, and c is added
+// back into the first k rows of its output.
+
+namespace {
+
+constexpr int kMaxDefl = 8;
+constexpr int kDeflCols = 256;  // columns (threads) per workgroup
+constexpr int kDeflU = 8;       // rows whose loads are in flight together
+
+struct DeflSeg {
+  int64_t off;       // first element of the segment (rows of n, row stride n)
+  int64_t u_off;     // U of the segment: ndet x k, row-major
+  int64_t c_off;     // c of the segment: k x n
+  int64_t blk_base;  // first workgroup of the segment
+  int32_t ndet, n, k, pad;
+};
+
+__device__ inline int find_dseg(const DeflSeg* segs, int nseg, int64_t b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].blk_base <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One workgroup per 256 sample columns of one segment (so the segment and U
+// are uniform per workgroup), one thread per column: c = U^T x over the
+// column (FP64), then out = x - U c, kDeflU rows' loads in flight at a time.
+// out may equal in (each thread reads its own column before writing it).
+__global__ void __launch_bounds__(kDeflCols) defl_kernel(const DeflSeg* __restrict__ segs, int nseg, const float* in,
+                                                         float* out, const double* __restrict__ U,
+                                                         double* __restrict__ C) {
+  const DeflSeg s = segs[find_dseg(segs, nseg, blockIdx.x)];
+  const int64_t t = (int64_t)(blockIdx.x - s.blk_base) * kDeflCols + threadIdx.x;
+  if (t >= s.n) return;
+  const float* x = in + s.off + t;
+  float* y = out + s.off + t;
+  const double* u = U + s.u_off;
+  const int ndet = s.ndet, k = s.k;
+  const int64_t n = s.n;
+  double acc[kMaxDefl];
+#pragma unroll
+  for (int j = 0; j < kMaxDefl; j++) acc[j] = 0.0;
+  int i = 0;
+  for (; i + kDeflU <= ndet; i += kDeflU) {
+    float xv[kDeflU];
+#pragma unroll
+    for (int q = 0; q < kDeflU; q++) xv[q] = x[(int64_t)(i + q) * n];
+#pragma unroll
+    for (int q = 0; q < kDeflU; q++)
+#pragma unroll
+      for (int j = 0; j < kMaxDefl; j++)
+        if (j < k) acc[j] += u[(i + q) * k + j] * (double)xv[q];
+  }
+  for (; i < ndet; i++) {
+    const double xv = (double)x[(int64_t)i * n];
+#pragma unroll
+    for (int j = 0; j < kMaxDefl; j++)
+      if (j < k) acc[j] += u[i * k + j] * xv;
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxDefl; j++)
+    if (j < k) C[s.c_off + (int64_t)j * n + t] = acc[j];
+  i = 0;
+  for (; i + kDeflU <= ndet; i += kDeflU) {
+    float xv[kDeflU];
+#pragma unroll
+    for (int q = 0; q < kDeflU; q++) xv[q] = x[(int64_t)(i + q) * n];
+#pragma unroll
+    for (int q = 0; q < kDeflU; q++) {
+      double r = (double)xv[q];
+#pragma unroll
+      for (int j = 0; j < kMaxDefl; j++)
+        if (j < k) r -= u[(i + q) * k + j] * acc[j];
+      y[(int64_t)(i + q) * n] = (float)r;
+    }
+  }
+  for (; i < ndet; i++) {
+    double r = (double)x[(int64_t)i * n];
+#pragma unroll
+    for (int j = 0; j < kMaxDefl; j++)
+      if (j < k) r -= u[i * k + j] * acc[j];
+    y[(int64_t)i * n] = (float)r;
+  }
+}
+
+// y[j][t] += c[j][t] for the first k rows of every segment.
+__global__ void __launch_bounds__(kDeflCols) defl_addback_kernel(const DeflSeg* __restrict__ segs, int nseg,
+                                                                 float* __restrict__ Y, const double* __restrict__ C) {
+  const DeflSeg s = segs[find_dseg(segs, nseg, blockIdx.x)];
+  const int64_t t = (int64_t)(blockIdx.x - s.blk_base) * kDeflCols + threadIdx.x;
+  if (t >= s.n) return;
+  for (int j = 0; j < s.k; j++) {
+    float* y = Y + s.off + (int64_t)j * s.n + t;
+    *y = (float)((double)*y + C[s.c_off + (int64_t)j * s.n + t]);
+  }
+}
+
+}  // namespace
+
+struct mk_defl_plan {
+  int nseg = 0;
+  int64_t ncols = 0, nblk = 0;
+  DeflSeg* d_segs = nullptr;
+  double* u = nullptr;
+  double* c = nullptr;
+};
+
+extern "C" int mk_defl_plan_destroy(mk_defl_plan* p) {
+  if (p) {
+    if (p->d_segs) (void)hipFree(p->d_segs);
+    if (p->u) (void)hipFree(p->u);
+    if (p->c) (void)hipFree(p->c);
+    delete p;
+  }
+  return MK_OK;
+}
+
+extern "C" int mk_defl_plan_create(int nseg, const int64_t* desc, const double* u_host, mk_defl_plan** out) {
+  MK_REQUIRE(out, MK_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  MK_REQUIRE(nseg >= 0 && (nseg == 0 || (desc && u_host)), MK_ERR_ARG, "bad segment table");
+  std::vector<DeflSeg> segs(nseg);
+  int64_t ncols = 0, nu = 0, nc = 0, nblk = 0;
+  for (int i = 0; i < nseg; i++) {
+    const int64_t* d = desc + 4 * i;
+    MK_REQUIRE(d[0] > 0 && d[0] < (1 << 20) && d[1] > 0 && d[1] < (1 << 30) && d[2] >= 0 && d[3] >= 1 &&
+                   d[3] <= kMaxDefl && d[3] <= d[0],
+               MK_ERR_ARG, "segment %d: need 1 <= k <= min(ndet, %d), sizes > 0, offset >= 0", i, kMaxDefl);
+    DeflSeg& s = segs[i];
+    s.ndet = (int32_t)d[0]; s.n = (int32_t)d[1]; s.off = d[2]; s.k = (int32_t)d[3]; s.pad = 0;
+    s.u_off = nu; s.c_off = nc; s.blk_base = nblk;
+    nu += (int64_t)s.ndet * s.k;
+    nc += (int64_t)s.k * s.n;
+    ncols += s.n;
+    nblk += mk::cdiv(s.n, kDeflCols);
+  }
+  MK_REQUIRE(nblk < INT32_MAX, MK_ERR_ARG, "deflation plan: too many columns");
+  mk_defl_plan* p = new mk_defl_plan();
+  p->nseg = nseg;
+  p->ncols = ncols;
+  p->nblk = nblk;
+  if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(DeflSeg)) != hipSuccess ||
+      hipMalloc((void**)&p->u, std::max<int64_t>(1, nu) * 8) != hipSuccess ||
+      hipMalloc((void**)&p->c, std::max<int64_t>(1, nc) * 8) != hipSuccess) {
+    mk_defl_plan_destroy(p);
+    mk::set_error("deflation plan: device allocation failed");
+    return MK_ERR_ALLOC;
+  }
+  if ((nseg && hipMemcpy(p->d_segs, segs.data(), nseg * sizeof(DeflSeg), hipMemcpyHostToDevice) != hipSuccess) ||
+      (nu && hipMemcpy(p->u, u_host, nu * 8, hipMemcpyHostToDevice) != hipSuccess)) {
+    mk_defl_plan_destroy(p);
+    mk::set_error("deflation plan: upload failed");
+    return MK_ERR_HIP;
+  }
+  *out = p;
+  return MK_OK;
+}
+
+extern "C" int mk_defl_plan_exec(const mk_defl_plan* p, const float* in, float* out, void* stream) {
+  MK_REQUIRE(p, MK_ERR_ARG, "null plan");
+  if (p->ncols == 0) return MK_OK;
+  MK_REQUIRE(in && out, MK_ERR_ARG, "null pointer");
+  defl_kernel<<<(unsigned)p->nblk, kDeflCols, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, in, out, p->u, p->c);
+  return mk::check_launch("defl_kernel");
+}
+
+extern "C" int mk_defl_plan_addback(const mk_defl_plan* p, float* y, void* stream) {
+  MK_REQUIRE(p, MK_ERR_ARG, "null plan");
+  if (p->ncols == 0) return MK_OK;
+  MK_REQUIRE(y, MK_ERR_ARG, "null pointer");
+  defl_addback_kernel<<<(unsigned)p->nblk, kDeflCols, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, y, p->c);
+  return mk::check_launch("defl_addback_kernel");
+}

@@ -119,6 +119,10 @@ _SIGS = {
     "mk_pix_tile_remap": ([P, I64, I32, I32, I32, I32, P, P, P], I),
     "mk_map_to_tiles_f32": ([P, I32, I32, I32, I32, P, P], I),
     "mk_map_to_tiles_f64": ([P, I32, I32, I32, I32, P, P], I),
+    "mk_defl_plan_create": ([I, P, P, P], I),
+    "mk_defl_plan_exec": ([P, P, P, P], I),
+    "mk_defl_plan_addback": ([P, P, P], I),
+    "mk_defl_plan_destroy": ([P], I),
     "mk_tod2cuts_f32": ([P, P, I64, P, I64, I, P, P], I),
     "mk_tod2cuts_f64": ([P, P, I64, P, I64, I, P, P], I),
     "mk_cuts2tod_f32": ([P, P, I64, P, I64, I, P, P], I),

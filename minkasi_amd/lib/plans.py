@@ -169,3 +169,35 @@ class RotPlan(_Handle):
     def exec(self, b, c):
         mkgpu.check("mk_rot_plan_exec", self.h, dev.ptr(b, torch.float32, "B"), dev.ptr(c, torch.float32, "C"),
                     dev.stream_ptr())
+
+
+class DeflPlan(_Handle):
+    """Deflation of dominant detector modes ahead of the FP32 rotation
+    (mk_defl_plan_*): exec(x, out) writes out = x - U U^T x per segment and
+    keeps c = U^T x; addback(y) adds c into the first k rows of y = v.out."""
+
+    _destroy = "mk_defl_plan_destroy"
+
+    def __init__(self, segs):
+        """segs: list of (ndet, n, off, U) with U an ndet x k FP64 array."""
+        super().__init__()
+        rows, us = [], []
+        for ndet, n, off, U in segs:
+            U = np.ascontiguousarray(U, dtype=np.float64)
+            if U.ndim != 2 or U.shape[0] != ndet:
+                raise ValueError("U must be ndet x k")
+            rows.append([int(ndet), int(n), int(off), int(U.shape[1])])
+            us.append(U.ravel())
+        self.nseg = len(rows)
+        desc = np.asarray(rows, dtype=np.int64).reshape(-1)
+        u = np.concatenate(us) if us else np.zeros(1)
+        if desc.size == 0:
+            desc = np.zeros(4, dtype=np.int64)
+        mkgpu.check("mk_defl_plan_create", len(rows), desc.ctypes.data, u.ctypes.data, ctypes.byref(self.h))
+
+    def exec(self, x, out):
+        mkgpu.check("mk_defl_plan_exec", self.h, dev.ptr(x, torch.float32, "in"), dev.ptr(out, torch.float32, "out"),
+                    dev.stream_ptr())
+
+    def addback(self, y):
+        mkgpu.check("mk_defl_plan_addback", self.h, dev.ptr(y, torch.float32, "y"), dev.stream_ptr())

@@ -23,7 +23,7 @@ import torch
 
 from .. import device as dev
 from ..lib import mkgpu
-from ..lib.plans import FiltPlan, RotPlan, T2MPlan
+from ..lib.plans import DeflPlan, FiltPlan, RotPlan, T2MPlan
 from .map2tod import map2tod
 from .noise import NoiseSmoothedSVD
 
@@ -111,6 +111,11 @@ class BatchedDot:
         self.rot_v = RotPlan([(tod.noise.v_split, n, o, n, o, n) for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
         self.rot_vT = RotPlan([(tod.noise.vT_split, n, o, n, o, n)
                                for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
+        # deflation of dominant detector modes ahead of the first rotation, for
+        # the TODs whose noise model asks for it
+        dsegs = [(ndet, n, o, tod.noise.v[:tod.noise.defl_k].T)
+                 for tod, o, (ndet, n) in zip(self.tods, offs, sizes) if getattr(tod.noise, "defl_k", 0)]
+        self.defl = DeflPlan(dsegs) if dsegs else None
         for tod in self.tods:  # per-TOD filter work buffers are not needed on the batched path
             tod.noise.filt32.release_work()
         self.graph = None
@@ -135,6 +140,16 @@ class BatchedDot:
             self.map_t = torch.zeros(size, dtype=torch.float32, device=self.ipix.device)
         self.gather_tile = (tx, ty)
         self.graph = None
+
+    def _rot_v(self, a, b):
+        """b = v.a per TOD (a is overwritten by its deflated copy when some
+        TOD's modes are deflated)."""
+        if self.defl is not None:
+            self.defl.exec(a, a)
+            self.rot_v.exec(a, b)
+            self.defl.addback(b)
+        else:
+            self.rot_v.exec(a, b)
 
     def _map2tod(self, x, a):
         """a = x.flat[ipix] over the arena, gathered from the tiled copy."""
@@ -164,7 +179,7 @@ class BatchedDot:
         n = self.nsamp
         a, b = self.buf_a[:n], self.buf_b[:n]
         self._map2tod(x, a)
-        self.rot_v.exec(a, b)
+        self._rot_v(a, b)
         self.filt.exec(b, b)
         self.rot_vT.exec(b, a)
         self.t2m.exec(out.reshape(-1), a, accumulate)
@@ -201,7 +216,7 @@ class BatchedDot:
             d = tod.get_data()
             d = d if isinstance(d, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(d))
             a[o:o + ndet * nn].copy_(d.reshape(-1).to(a.device, torch.float32))
-        self.rot_v.exec(a, b)
+        self._rot_v(a, b)
         self.filt.exec(b, b)
         self.rot_vT.exec(b, a)
         self.t2m.exec(out.reshape(-1), a, accumulate)
@@ -220,7 +235,7 @@ class BatchedDot:
         a, b = self.buf_a[:n], self.buf_b[:n]
         steps = [
             ("map2tod", lambda: self._map2tod(x, a)),
-            ("rot_v", lambda: self.rot_v.exec(a, b)),
+            ("rot_v", lambda: self._rot_v(a, b)),
             ("filter", lambda: self.filt.exec(b, b)),
             ("rot_vT", lambda: self.rot_vT.exec(b, a)),
             ("tod2map", lambda: self.t2m.exec(out.reshape(-1), a, False)),
@@ -364,7 +379,7 @@ class BatchedCutsDot:
         xm = x[:self.npix].view(B.nx, B.ny)
         B._map2tod(xm, a)
         self._cuts_in(x[self.npix:], a)
-        B.rot_v.exec(a, b)
+        B._rot_v(a, b)
         B.filt.exec(b, b)
         B.rot_vT.exec(b, a)
         B.t2m.exec(out[:self.npix], a, accumulate)

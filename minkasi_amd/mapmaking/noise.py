@@ -20,7 +20,7 @@ import torch
 
 from .. import device as dev
 from ..lib import mkgpu
-from ..lib.plans import FiltPlan, RotPlan, SplitMatrix
+from ..lib.plans import DeflPlan, FiltPlan, RotPlan, SplitMatrix
 from ..tools import fft as gfft
 from ..tools.smooth import smooth_rows
 
@@ -47,6 +47,28 @@ class WithDetWeights(NoiseModelType, Protocol):
 
     def get_det_weights(self):
         return None
+
+
+# A mode is deflated ahead of the FP32 rotation when its rotated row is more
+# than DEFLATE_RATIO times the median row norm: the FP32 accumulation error of
+# the small rows grows as about 1e-8 times the largest such ratio left in the
+# data, so 500 keeps apply_noise within about 5e-6 of FP64 while TODs whose
+# modes stay below it skip the extra pass (12 bytes per sample).
+DEFLATE_RATIO = 500.0
+MAX_DEFLATE = 8
+
+
+def deflation_rank(row_norms):
+    """Number of leading rows of v.d (modes in decreasing power) whose norm
+    exceeds DEFLATE_RATIO times the median row norm (at most MAX_DEFLATE)."""
+    row_norms = np.asarray(row_norms, dtype=np.float64)
+    med = float(np.median(row_norms)) if row_norms.size else 0.0
+    if not med > 0:
+        return 0
+    k = 0
+    while k < min(MAX_DEFLATE, row_norms.size - 1) and row_norms[k] > DEFLATE_RATIO * med:
+        k += 1
+    return k
 
 
 def _gram(d):
@@ -112,6 +134,10 @@ class NoiseSmoothedSVD:
         # spectra, FP64 on the device
         v_dev = torch.from_numpy(np.ascontiguousarray(self.v)).to(d.device)
         y = _gemm64(v_dev, d)                                # dat_rot (noise.py:667)
+        # modes that dominate the rotated data are deflated ahead of the FP32
+        # rotation (DeflPlan); the row norms of v.d are the modes' amplitudes
+        self.defl_k = 0 if self.noisevec is not None else deflation_rank(
+            torch.linalg.vector_norm(y, dim=1).cpu().numpy())
         Y = gfft._dct1_tensor(y)                              # dat_trans (noise.py:674)
         del y
         W, self.mywt = smooth_rows(Y, self.fwhm, square=True, invert=True, want_f32=True)  # noise.py:675-677
@@ -143,6 +169,8 @@ class NoiseSmoothedSVD:
         v, vT = self._rot_mats()
         self.v_split = SplitMatrix(v)
         self.vT_split = SplitMatrix(vT)
+        k = getattr(self, "defl_k", 0)
+        self.defl = DeflPlan([(ndet, n, 0, self.v[:k].T)]) if k else None
         self.rot_v = RotPlan([(self.v_split, n, 0, n, 0, n)])
         self.rot_vT = RotPlan([(self.vT_split, n, 0, n, 0, n)])
         d = dev.device()
@@ -164,7 +192,12 @@ class NoiseSmoothedSVD:
 
     def apply_noise_wscratch(self, dat, tmp, tmp2):
         """FP32 apply with caller buffers (noise.py:716-760); result in tmp2."""
-        self.rot_v.exec(dat, tmp)
+        if self.defl is not None:
+            self.defl.exec(dat, tmp2)
+            self.rot_v.exec(tmp2, tmp)
+            self.defl.addback(tmp)
+        else:
+            self.rot_v.exec(dat, tmp)
         self.filt32.exec(tmp, tmp)
         self.rot_vT.exec(tmp, tmp2)
         return tmp2

@@ -60,6 +60,10 @@ def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False,
             print("noise model", k, "of", len(todvec.tods), flush=True)
     torch.cuda.synchronize()
     P.t_noise = time.time() - t0
+    P.ndeflated = sum(1 for tod in todvec.tods if getattr(tod.noise, "defl_k", 0))
+    if verbose:
+        print("noise models: %d of %d TODs deflate dominant modes ahead of the FP32 rotation" % (
+            P.ndeflated, len(todvec.tods)), flush=True)
     t0 = time.time()
     mapset = Mapset()
     mapset.add_map(m)

@@ -1,0 +1,114 @@
+"""Precision of the FP32 path on the precision workload (1024 detectors, 20%
+of them in a rank-5 common mode of 10x the white-noise amplitude): the
+split-BF16 rotation, apply_noise and tod2map against the FP64 CPU oracle,
+and the map after PCG against the FP64 device path.  The FP64 device path
+is itself pinned to the oracle (test_gpu_api.py::test_fp64_config0_path_
+matches_oracle) and is the reference where the oracle is too slow (many
+PCG iterations over 2e9 samples).
+
+By default the TOD is shortened to 32768 samples (1 TOD, 20 iterations).
+MK_FULL_PRECISION=1 runs the full workload (8 TODs x 1024 x 262144, 100
+iterations); MK_PRECISION_REPORT=<path> writes the measured errors as JSON."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import core
+from oracle import noise as onoise
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+FULL = os.environ.get("MK_FULL_PRECISION") == "1"
+
+
+def _stage_errors(P, k=0):
+    import torch
+    tod = P.todvec.tods[k]
+    d = tod.get_data()
+    d64 = d.to(torch.float64).cpu().numpy()
+    v = tod.noise.v
+    rot = torch.empty_like(d)
+    tod.noise.rot_v.exec(d, rot)
+    e_rot = rel_err(rot.cpu().numpy(), v @ d64)
+    onz = onoise.NoiseSmoothedSVD(d64, fwhm=tod.noise.fwhm, u_in=v.T.copy())  # same basis as the device model
+    e_an = rel_err(tod.noise.apply_noise(d).cpu().numpy(), onz.apply_noise(d64))
+    return e_rot, e_an
+
+
+def _tod2map_error(P):
+    import torch
+    op = P.op
+    n = op.nsamp
+    a = op.buf_a[:n]
+    for tod, o, (ndet, nn) in zip(op.tods, op.offs, op.sizes):
+        a[o:o + ndet * nn].copy_(tod.get_data().reshape(-1))
+    out = torch.zeros(op.nx * op.ny, dtype=torch.float32, device=a.device)
+    op.t2m.exec(out, a, False)
+    ref = np.zeros(op.nx * op.ny)
+    core.tod2map(ref, a.to(torch.float64).cpu().numpy().reshape(1, n), op.ipix[:n].cpu().numpy().reshape(1, n))
+    return rel_err(out.cpu().numpy(), ref)
+
+
+def _maps_after(P, iters):
+    """FP32 batched Jacobi PCG and the FP64 per-TOD path, `iters` iterations
+    each from x0 = 0; both share each TOD's detector basis."""
+    import torch
+    import minkasi_amd as mk
+    from minkasi_amd.mapmaking.pcg import DevicePCG
+    from minkasi_amd.workloads import map_distance
+    solver = DevicePCG(P.op, P.rhs.maps[0].map, P.precon)
+    x32 = torch.zeros_like(P.rhs.maps[0].map)
+    solver.init(x32)
+    for _ in range(iters):
+        solver.step()
+    tv = mk.TodVec()
+    for tod in P.todvec.tods:
+        info = {"dat_calib": tod.get_data().to(torch.float64), "ipix": tod.info["ipix"]}
+        tv.add_tod(mk.Tod(info))
+    for t64, tod in zip(tv.tods, P.todvec.tods):
+        t64.set_noise(mk.NoiseSmoothedSVD, fwhm=tod.noise.fwhm, u_in=tod.noise.v.T.copy(), keep_fp64=True)
+    m64 = P.map.copy()
+    m64.map = m64.map.to(torch.float64)
+    ms = mk.Mapset()
+    ms.add_map(m64)
+    rhs = ms.copy()
+    rhs.clear()
+    tv.make_rhs(rhs)
+    e_rhs = rel_err(P.rhs.maps[0].map.cpu().numpy(), rhs.maps[0].map.cpu().numpy())
+    pre = ms.copy()
+    pre.maps[0].map.copy_(P.precon.maps[0].map.to(torch.float64))
+    x0 = rhs.copy()
+    x0.clear()
+    x64 = mk.run_pcg(rhs, x0, tv, pre, maxiter=iters, verbose=FULL).maps[0].map  # progress lines on long runs
+    for t64 in tv.tods:  # free the FP64 filter work buffers
+        t64.noise.filt64.release_work()
+    return e_rhs, map_distance(P, x32, x64), solver.rel_res()
+
+
+def test_precision_workload_errors():
+    from minkasi_amd.workloads import build_problem
+    if FULL:
+        P = build_problem(3, verbose=True)
+        iters = 100
+    else:
+        P = build_problem(3, shapes=[(1024, 32768)])
+        iters = 20
+    e_rot, e_an = _stage_errors(P)
+    e_t2m = _tod2map_error(P)
+    e_rhs, e_map, rel = _maps_after(P, iters)
+    rep = {"workload": "precision: %d TODs of %s, 20%% of detectors in a rank-5 common mode (10x white)" % (
+        len(P.shapes), "x".join(map(str, P.shapes[0]))), "iterations": iters,
+        "rotation_rel_err": e_rot, "apply_noise_rel_err": e_an, "tod2map_rel_err": e_t2m,
+        "rhs_rel_err_vs_fp64_device": e_rhs, "map_rel_err_vs_fp64_device": e_map, "fp32_rel_res": rel}
+    print(json.dumps(rep))
+    out = os.environ.get("MK_PRECISION_REPORT")
+    if out:
+        with open(out, "w") as f:
+            json.dump(rep, f, indent=1)
+    assert e_rot < 1e-6       # six split-BF16 products: ~FP32 rounding
+    assert e_an < 1e-5        # the apply_noise tolerance
+    assert e_t2m < 1e-5
+    assert e_rhs < 1e-5
+    assert e_map < 1e-2       # CG iterates amplify the FP32 operator error; the value is stated in the report
