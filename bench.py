@@ -206,6 +206,8 @@ def main():
     P = build_problem(args.config, ntod=args.ntod, verbose=True, truncate=args.truncate)
     log("setup: generate %.1fs pixelize %.1fs noise %.1fs plans %.1fs rhs %.1fs; %d TODs, %.3g samples" % (
         P.t_generate, P.t_pixelize, P.t_noise, P.t_plans, P.t_rhs, len(P.shapes), P.nsamp))
+    log("operator memory (GB):", json.dumps({k: (round(v / 1e9, 2) if k != "filter_groups" else v)
+                                              for k, v in P.op.memory_report().items()}))
     op = P.op
     b = P.rhs.maps[0].map
     ncut = 0
@@ -241,9 +243,11 @@ def main():
     st = P.op.stage_times(pm, torch.empty_like(pm), reps=3)
     sb = stage_bytes(P)
     dom = max(st, key=st.get)
+    # the committed traffic counts are of the default workload only
+    profiled = (args.config, args.ntod, args.truncate, args.cuts) == (4, None, False, 0.0)
     achieved = sb[dom] / (st[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom) if profiled else None,
             "algorithmic_bytes": int(sb[dom]), "stage_ms": {k: round(v, 3) for k, v in st.items()},
             "stage_frac": {k: round(sb[k] / (st[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k in st}}
     log("stage ms:", json.dumps(roof["stage_ms"]), "frac:", json.dumps(roof["stage_frac"]))
@@ -265,6 +269,9 @@ def main():
         "setup_s": round(t_setup, 1),
         "roofline": roof,
     }
+    free, total = torch.cuda.mem_get_info()
+    result["config"]["memory"] = {"hbm_used_gb": round((total - free) / 1e9, 1), "hbm_gb": round(total / 1e9, 1),
+                                  "tiled_gather": bool(P.op.tiled), "filter_groups": len(P.op.filts)}
     if tuner is not None:
         result["config"]["tuning"] = dict(tuner.report(), fft_measure_s=round(t_tune, 1))
     if args.profile:

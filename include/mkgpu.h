@@ -220,10 +220,12 @@ int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
  * desc (host): 6 int64 per segment (TOD): rows, n, in_off, in_stride,
  *   out_off, out_stride (element offsets into the exec in/out arrays).
  * elem_bytes: 4 (FP32 transform) or 8 (FP64 transform).
- * The plan owns its spectra (allocated at creation) and a work buffer of
+ * The plan owns its spectra (allocated by mk_filt_plan_set_weights; a plan
+ * whose segments all borrow spectra holds none) and a work buffer of
  * rows*P complex values, allocated by the first call that needs it (so
  * capture a graph only after one eager exec) and freed by
- * mk_filt_plan_release_work. */
+ * mk_filt_plan_release_work, or supplied by the caller
+ * (mk_filt_plan_set_work). */
 typedef struct mk_filt_plan mk_filt_plan;
 int mk_filt_plan_create(int nseg, const int64_t *desc, int elem_bytes,
                         mk_filt_plan **plan);
@@ -253,8 +255,20 @@ int mk_filt_plan_borrow_spectrum(mk_filt_plan *plan, int seg,
 int mk_filt_plan_set_halve(mk_filt_plan *plan, int halve_edges);
 int mk_filt_plan_exec_f32(const mk_filt_plan *plan, const float *in,
                           float *out, void *stream);
-/* Frees the work buffer (synchronises the stream first). */
+/* Frees the work buffer (synchronises the stream first); a caller's buffer
+ * (mk_filt_plan_set_work) is only detached. */
 int mk_filt_plan_release_work(mk_filt_plan *plan, void *stream);
+/* Bytes of work buffer the plan needs (rows * P complex values). */
+int mk_filt_plan_work_bytes(const mk_filt_plan *plan, int64_t *bytes);
+/* Makes the plan use the caller's device buffer (>= mk_filt_plan_work_bytes,
+ * 16-byte aligned) as its work buffer instead of one of its own; NULL
+ * detaches it.  Plans run one after another on one stream can share a
+ * buffer, which bounds the work memory of a large batch: the batched
+ * operator splits its TODs into groups that way.  No counterpart in the
+ * reference, whose FFTs use per-call numpy/FFTW buffers
+ * (minkasi/tools/fft.py:50-124). */
+int mk_filt_plan_set_work(mk_filt_plan *plan, void *work, int64_t bytes,
+                          void *stream);
 int mk_filt_plan_exec_f64(const mk_filt_plan *plan, const double *in,
                           double *out, void *stream);
 /* info[0..7] = nseg, K1/K3 items, K2 items, work bytes, spectrum bytes,

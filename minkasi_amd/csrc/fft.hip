@@ -1297,6 +1297,7 @@ struct mk_filt_plan {
   int64_t n1 = 0, n2 = 0;  // total K1/K3 items, K2 items
   int64_t t_elems = 0, s_elems = 0;
   void* tbuf = nullptr;    // complex
+  bool tbuf_owned = true;  // false: the caller's buffer (mk_filt_plan_set_work)
   void* spec = nullptr;    // real, P+1 per row
   size_t lds1 = 0, lds2 = 0;
   bool have_spec = false;
@@ -1322,6 +1323,7 @@ namespace {
 int ensure_tbuf(const mk_filt_plan* cp) {
   mk_filt_plan* p = const_cast<mk_filt_plan*>(cp);
   if (p->tbuf) return MK_OK;
+  p->tbuf_owned = true;
   if (hipMalloc(&p->tbuf, std::max<int64_t>(1, p->t_elems) * 2 * p->elem_bytes) != hipSuccess) {
     p->tbuf = nullptr;
     mk::set_error("filter plan: work buffer allocation (%lld B) failed", (long long)(p->t_elems * 2 * p->elem_bytes));
@@ -1444,7 +1446,7 @@ void free_filt(mk_filt_plan* p) {
   if (p->d_hot13) (void)hipFree(p->d_hot13);
   if (p->d_cold) (void)hipFree(p->d_cold);
   if (p->tw_tab) (void)hipFree(p->tw_tab);
-  if (p->tbuf) (void)hipFree(p->tbuf);
+  if (p->tbuf && p->tbuf_owned) (void)hipFree(p->tbuf);
   if (p->spec) (void)hipFree(p->spec);
   delete p;
 }
@@ -1594,13 +1596,14 @@ extern "C" int mk_filt_plan_create_p1(int nseg, const int64_t* desc, int elem_by
   if (hipMalloc((void**)&p->d_segs, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
       hipMalloc((void**)&p->d_hot, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
       hipMalloc((void**)&p->d_cold, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
-      hipMalloc((void**)&p->d_hot13, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess ||
-      hipMalloc(&p->spec, std::max<int64_t>(1, p->s_elems) * elem_bytes) != hipSuccess) {
+      hipMalloc((void**)&p->d_hot13, std::max<size_t>(1, nseg) * sizeof(Seg)) != hipSuccess) {
     free_filt(p);
     mk::set_error("filter plan: device allocation failed");
     return MK_ERR_ALLOC;
   }
-  for (auto& sg : p->h_segs) sg.s_ptr = (unsigned char*)p->spec + sg.s_off * elem_bytes;
+  // spectra are allocated by mk_filt_plan_set_weights: a plan that borrows
+  // every segment's spectrum never holds any of its own
+  for (auto& sg : p->h_segs) sg.s_ptr = nullptr;
   if (hot) {
     st = build_hot_tables(p);
     if (st) { free_filt(p); return st; }
@@ -1667,6 +1670,16 @@ extern "C" int mk_filt_plan_set_weights(mk_filt_plan* p, const double* wdct, voi
     MK_REQUIRE(!p->borrowed[i], MK_ERR_ARG, "plan borrows spectra; set_weights would overwrite the source plan");
   if (p->n1 == 0) { p->have_spec = true; return MK_OK; }
   hipStream_t st = mk::as_stream(stream);
+  if (!p->spec) {
+    if (hipMalloc(&p->spec, std::max<int64_t>(1, p->s_elems) * p->elem_bytes) != hipSuccess) {
+      p->spec = nullptr;
+      mk::set_error("filter plan: spectrum allocation (%lld B) failed", (long long)(p->s_elems * p->elem_bytes));
+      return MK_ERR_ALLOC;
+    }
+    for (auto& sg : p->h_segs) sg.s_ptr = (unsigned char*)p->spec + sg.s_off * p->elem_bytes;
+    int r = upload(p);
+    if (r) return r;
+  }
   // The spectrum is always computed in FP64 (a transient FP64 work buffer)
   // and stored in the plan's precision, unless FP64 LDS tiles would not fit.
   size_t l1_64 = 0, l2_64 = 0;
@@ -1718,8 +1731,28 @@ extern "C" int mk_filt_plan_release_work(mk_filt_plan* p, void* stream) {
   MK_REQUIRE(p, MK_ERR_ARG, "null plan");
   if (p->tbuf) {
     MK_HIP(hipStreamSynchronize(mk::as_stream(stream)));
-    MK_HIP(hipFree(p->tbuf));
+    if (p->tbuf_owned) MK_HIP(hipFree(p->tbuf));
     p->tbuf = nullptr;
+  }
+  return MK_OK;
+}
+
+extern "C" int mk_filt_plan_work_bytes(const mk_filt_plan* p, int64_t* bytes) {
+  MK_REQUIRE(p && bytes, MK_ERR_ARG, "null pointer");
+  *bytes = p->t_elems * 2 * p->elem_bytes;
+  return MK_OK;
+}
+
+extern "C" int mk_filt_plan_set_work(mk_filt_plan* p, void* work, int64_t bytes, void* stream) {
+  MK_REQUIRE(p, MK_ERR_ARG, "null plan");
+  const int64_t need = p->t_elems * 2 * p->elem_bytes;
+  MK_REQUIRE(work == nullptr || (bytes >= need && ((uintptr_t)work & 15) == 0), MK_ERR_ARG,
+             "work buffer of %lld B (need %lld B, 16-byte aligned)", (long long)bytes, (long long)need);
+  int r = mk_filt_plan_release_work(p, stream);
+  if (r) return r;
+  if (work) {
+    p->tbuf = work;
+    p->tbuf_owned = false;
   }
   return MK_OK;
 }

@@ -88,6 +88,8 @@ _SIGS = {
     "mk_filt_plan_exec_f32": ([P, P, P, P], I),
     "mk_filt_plan_exec_f64": ([P, P, P, P], I),
     "mk_filt_plan_release_work": ([P, P], I),
+    "mk_filt_plan_work_bytes": ([P, P], I),
+    "mk_filt_plan_set_work": ([P, P, I64, P], I),
     "mk_filt_plan_info": ([P, P], I),
     "mk_filt_plan_geometry": ([P, P], I),
     "mk_filt_plan_destroy": ([P], I),

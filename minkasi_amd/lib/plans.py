@@ -102,6 +102,21 @@ class FiltPlan(_Handle):
 
     def release_work(self):
         mkgpu.check("mk_filt_plan_release_work", self.h, dev.stream_ptr())
+        self._work = None
+
+    @property
+    def work_bytes(self):
+        out = ctypes.c_int64()
+        mkgpu.check("mk_filt_plan_work_bytes", self.h, ctypes.byref(out))
+        return out.value
+
+    def set_work(self, buf):
+        """Use the device tensor buf (at least work_bytes bytes) as the work
+        buffer; plans run one after another on one stream may share one."""
+        nbytes = buf.numel() * buf.element_size()
+        mkgpu.check("mk_filt_plan_set_work", self.h, ctypes.c_void_p(buf.data_ptr()), int(nbytes),
+                    dev.stream_ptr())
+        self._work = buf
 
     def borrow(self, seg, src, src_seg=0):
         mkgpu.check("mk_filt_plan_borrow_spectrum", self.h, int(seg), src.h, int(src_seg))

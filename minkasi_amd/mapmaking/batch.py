@@ -32,6 +32,47 @@ from .noise import NoiseSmoothedSVD
 # first is the default, the autotuner times all of them
 GATHER_TILES = ((8, 4), (16, 2), (4, 8), (32, 1))
 
+# Device memory left free once the operator is built (the solver's map
+# vectors, the caller's temporaries).
+MEM_RESERVE = 6 << 30
+# Work buffer below which the filter would rather give up the tiled gather's
+# index copy than be split into more groups.
+MIN_FILTER_WORK = 8 << 30
+
+
+def filter_groups(work, cap):
+    """Consecutive TODs grouped so that each group's filter work buffer
+    (sum of its TODs' work bytes) stays within cap, with the groups about
+    equal.  Returns lists of TOD indices."""
+    if not work:
+        return []
+    total, top = sum(work), max(work)
+    if cap < top:
+        raise MemoryError("the filter needs %.2f GB of work memory for its largest TOD, %.2f GB are left"
+                          % (top / 1e9, cap / 1e9))
+    if total <= cap:
+        return [list(range(len(work)))]
+    ngroups = -(-total // cap)
+    target = min(cap, max(top, -(-total // ngroups)))
+    groups, cur, acc = [], [], 0
+    for i, w in enumerate(work):
+        if cur and acc + w > target:
+            groups.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += w
+    groups.append(cur)
+    if max(sum(work[i] for i in g) for g in groups) > cap:  # target overshoot: plain greedy fill to cap
+        groups, cur, acc = [], [], 0
+        for i, w in enumerate(work):
+            if cur and acc + w > cap:
+                groups.append(cur)
+                cur, acc = [], 0
+            cur.append(i)
+            acc += w
+        groups.append(cur)
+    return groups
+
 
 def _hot_p1(plan):
     """The FFT column length of a one-segment FP32 filter plan when it is on
@@ -57,7 +98,13 @@ class BatchedDot:
                 return False
         return True
 
-    def __init__(self, todvec, skymap, tile=(64, 32), item_samples=None):
+    def __init__(self, todvec, skymap, tile=(64, 32), item_samples=None, tiled_gather=None, filter_work_cap=None):
+        """tiled_gather: gather map2tod from a tiled copy of the map through a
+        second index array (4 bytes per sample), or from the row-major map
+        (False); None decides from the free device memory.
+        filter_work_cap: largest filter work buffer in bytes (None: what the
+        free device memory allows).  TODs whose filter work exceeds it in
+        total are filtered in groups that share one buffer."""
         self.tods = list(todvec.tods)
         self.nx, self.ny, self.tag = skymap.nx, skymap.ny, skymap.tag
         d = dev.device()
@@ -80,13 +127,26 @@ class BatchedDot:
         self.ipix_ids = [id(tod.info[self.tag]) for tod in self.tods]
         self.buf_a = torch.empty(max(off, 4), dtype=torch.float32, device=d)
         self.buf_b = torch.empty(max(off, 4), dtype=torch.float32, device=d)
+        # the per-TOD index blocks the arena replaced go back to the device,
+        # where the library's own allocations can use them
+        torch.cuda.empty_cache()
         from .. import tune
         tuner = tune.active()
-        # gather indices into a tiled copy of the map (see _set_gather_tile)
-        self.ipix_g = torch.empty(max(off, 4), dtype=torch.int32, device=d)
+        # the tod2map plan first: its build needs sort buffers over 2^30-sample
+        # segments on top of the 6 bytes per sample it keeps
+        if tuner is not None and off > 0 and tile == (64, 32) and item_samples is None:
+            self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
+        else:
+            self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
+        work = [int(tod.noise.filt32.work_bytes) for tod in self.tods]
+        self.tiled, cap = self._memory_plan(work, off, tiled_gather, filter_work_cap)
+        self.ipix_g = None
         self.map_t = None
         self.gather_tile = None
-        if tuner is not None and off > 0:
+        if self.tiled:
+            # gather indices into a tiled copy of the map (see _set_gather_tile)
+            self.ipix_g = torch.empty(max(off, 4), dtype=torch.int32, device=d)
+        if self.tiled and tuner is not None and off > 0:
             sig = tune.ipix_signature(self.ipix[:off], self.nx, self.ny)
             x = torch.randn(self.nx, self.ny, device=d)
 
@@ -96,18 +156,24 @@ class BatchedDot:
 
             best = tuner.choose("gather_tile", sig, [list(t) for t in GATHER_TILES], time_tile)
             self._set_gather_tile(tuple(best))
-        else:
+        elif self.tiled:
             self._set_gather_tile(GATHER_TILES[0])
-        if tuner is not None and off > 0 and tile == (64, 32) and item_samples is None:
-            self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
-        else:
-            self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
-        segs = [(ndet, n, o, n, o, n) for o, (ndet, n) in zip(offs, sizes)]
-        # the same FFT column lengths as the noise models whose spectra are borrowed
-        self.filt = FiltPlan(segs, 4, p1=[_hot_p1(tod.noise.filt32) for tod in self.tods])
-        for i, tod in enumerate(self.tods):
-            self.filt.borrow(i, tod.noise.filt32, 0)
-        self.filt.set_halve(True)
+        # filter plans over groups of TODs (one group unless memory is short),
+        # with the same FFT column lengths as the noise models whose spectra
+        # they borrow, sharing one work buffer
+        self.filts = []
+        self.filter_groups = filter_groups(work, cap)
+        for grp in self.filter_groups:
+            segs = [(sizes[i][0], sizes[i][1], offs[i], sizes[i][1], offs[i], sizes[i][1]) for i in grp]
+            fp = FiltPlan(segs, 4, p1=[_hot_p1(self.tods[i].noise.filt32) for i in grp])
+            for k, i in enumerate(grp):
+                fp.borrow(k, self.tods[i].noise.filt32, 0)
+            fp.set_halve(True)
+            self.filts.append(fp)
+        wmax = max((sum(work[i] for i in g) for g in self.filter_groups), default=0)
+        self.filter_work = torch.empty(max(wmax, 16), dtype=torch.uint8, device=d)
+        for fp in self.filts:
+            fp.set_work(self.filter_work)
         self.rot_v = RotPlan([(tod.noise.v_split, n, o, n, o, n) for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
         self.rot_vT = RotPlan([(tod.noise.vT_split, n, o, n, o, n)
                                for tod, o, (ndet, n) in zip(self.tods, offs, sizes)])
@@ -120,6 +186,44 @@ class BatchedDot:
             tod.noise.filt32.release_work()
         self.graph = None
         self._gx = self._gout = None
+
+    @staticmethod
+    def _memory_plan(work, nsamp, tiled_gather, filter_work_cap):
+        """(tiled gather?, filter work cap in bytes) from the device memory
+        free once the tod2map plan is built: less a reserve, it goes to the
+        tiled gather's index copy (4 bytes per sample) and the filter work
+        buffer, and the index copy is given up first when the filter would
+        otherwise get less than MIN_FILTER_WORK (or its largest TOD's work)."""
+        if not work:
+            return bool(tiled_gather is not False), 0
+        free = int(torch.cuda.mem_get_info()[0])
+        left = free - MEM_RESERVE
+        total, top = sum(work), max(work)
+        if tiled_gather is None:
+            tiled_gather = left - 4 * nsamp >= max(top, min(total, MIN_FILTER_WORK))
+        if tiled_gather:
+            left -= 4 * nsamp
+        if left < top:
+            raise MemoryError(
+                "batched operator over %.3g samples: %.1f GB of device memory free after the tod2map plan, not "
+                "enough for %sa filter work buffer of %.1f GB and a %.1f GB reserve" % (
+                    nsamp, free / 1e9, "tiled gather indices (%.1f GB), " % (4 * nsamp / 1e9) if tiled_gather else "",
+                    top / 1e9, MEM_RESERVE / 1e9))
+        cap = left if filter_work_cap is None else min(left, int(filter_work_cap))
+        return bool(tiled_gather), cap
+
+    def _filter(self, b):
+        """b = C W C b in place, group by group."""
+        for fp in self.filts:
+            fp.exec(b, b)
+
+    def memory_report(self):
+        """Device bytes held by the operator's own buffers and plans."""
+        n = self.nsamp
+        t = self.t2m.info()
+        return {"work_tods": 8 * n, "ipix": 4 * n, "tiled_gather_ipix": 4 * n if self.tiled else 0,
+                "tod2map_plan": t["plan_bytes"] + t["partial_bytes"], "filter_work": int(self.filter_work.numel()),
+                "filter_groups": len(self.filts)}
 
     def _set_gather_tile(self, tile):
         """Rewrite the gather indices for tx x ty map tiles and size the
@@ -152,7 +256,11 @@ class BatchedDot:
             self.rot_v.exec(a, b)
 
     def _map2tod(self, x, a):
-        """a = x.flat[ipix] over the arena, gathered from the tiled copy."""
+        """a = x.flat[ipix] over the arena, gathered from the tiled copy (or
+        from the row-major map when memory is short)."""
+        if not self.tiled:
+            map2tod(a, x.reshape(-1), self.ipix[:self.nsamp], do_add=False)
+            return
         tx, ty = self.gather_tile
         mkgpu.check("mk_map_to_tiles_f32", dev.ptr(x, torch.float32, "map"), self.nx, self.ny, tx, ty,
                     self.map_t.data_ptr(), dev.stream_ptr())
@@ -180,7 +288,7 @@ class BatchedDot:
         a, b = self.buf_a[:n], self.buf_b[:n]
         self._map2tod(x, a)
         self._rot_v(a, b)
-        self.filt.exec(b, b)
+        self._filter(b)
         self.rot_vT.exec(b, a)
         self.t2m.exec(out.reshape(-1), a, accumulate)
 
@@ -217,7 +325,7 @@ class BatchedDot:
             d = d if isinstance(d, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(d))
             a[o:o + ndet * nn].copy_(d.reshape(-1).to(a.device, torch.float32))
         self._rot_v(a, b)
-        self.filt.exec(b, b)
+        self._filter(b)
         self.rot_vT.exec(b, a)
         self.t2m.exec(out.reshape(-1), a, accumulate)
         return out
@@ -236,7 +344,7 @@ class BatchedDot:
         steps = [
             ("map2tod", lambda: self._map2tod(x, a)),
             ("rot_v", lambda: self._rot_v(a, b)),
-            ("filter", lambda: self.filt.exec(b, b)),
+            ("filter", lambda: self._filter(b)),
             ("rot_vT", lambda: self.rot_vT.exec(b, a)),
             ("tod2map", lambda: self.t2m.exec(out.reshape(-1), a, False)),
         ]
@@ -380,7 +488,7 @@ class BatchedCutsDot:
         B._map2tod(xm, a)
         self._cuts_in(x[self.npix:], a)
         B._rot_v(a, b)
-        B.filt.exec(b, b)
+        B._filter(b)
         B.rot_vT.exec(b, a)
         B.t2m.exec(out[:self.npix], a, accumulate)
         self._cuts_out(a, out[self.npix:], accumulate)

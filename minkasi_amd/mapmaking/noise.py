@@ -95,7 +95,8 @@ class NoiseSmoothedSVD:
     """GPU noise model with the reference's constructor and methods.
 
     Attributes (reference names): v, vT (numpy FP64, ndet x ndet), mywt
-    (device FP32, ndet x n), noisevec (numpy or None).
+    (numpy FP32, ndet x n, in host memory), noisevec (numpy or None); and
+    mode_wt, the FP64 row sums of mywt.
     """
 
     def __init__(self, dat_use, fwhm=50.0, prewhiten=False, fit_powlaw=False, u_in=None, keep_fp64=None):
@@ -140,8 +141,13 @@ class NoiseSmoothedSVD:
             torch.linalg.vector_norm(y, dim=1).cpu().numpy())
         Y = gfft._dct1_tensor(y)                              # dat_trans (noise.py:674)
         del y
-        W, self.mywt = smooth_rows(Y, self.fwhm, square=True, invert=True, want_f32=True)  # noise.py:675-677
+        W = smooth_rows(Y, self.fwhm, square=True, invert=True)  # mywt in FP64 (noise.py:675-677)
         del Y
+        # get_det_weights needs only the row sums; mywt itself is kept in host
+        # memory (FP32), where the reference keeps its numpy array, so that it
+        # does not hold 4 bytes of HBM per sample
+        self.mode_wt = W.sum(dim=1).cpu().numpy()
+        self.mywt = W.to(torch.float32).cpu().numpy()
         wdct = gfft._dct1_tensor(W)
         del W
         self._build_apply(wdct, keep_fp64 if keep_fp64 is not None else (self.dtype == torch.float64))
@@ -211,6 +217,5 @@ class NoiseSmoothedSVD:
 
     def get_det_weights(self):
         """noise.py:762-772."""
-        mode_wt = self.mywt.to(torch.float64).sum(dim=1).cpu().numpy()
-        tmp = np.dot(self.vT, np.dot(np.diag(mode_wt), self.v))
+        tmp = np.dot(self.vT, np.dot(np.diag(self.mode_wt), self.v))
         return np.diag(tmp).copy() * 2.0

@@ -227,30 +227,27 @@ class Tuner:
             e1.synchronize()
             return e0.elapsed_time(e1) / reps
 
+        # one candidate plan alive at a time (a plan holds 6 bytes per
+        # sample); the winner is built again at the end
         best, best_ms, results = None, math.inf, []
         for tile in T2M_TILES:
             plan = T2MPlan(ipix, nx, ny, tile=tile)
             ms = timed(plan)
             results.append((tile, plan.item_samples, ms))
             if ms < best_ms:
-                if best is not None:
-                    best.close()
-                best, best_ms = plan, ms
-            else:
-                plan.close()
-        base = best.item_samples
+                best, best_ms = (tuple(tile), plan.item_samples), ms
+            plan.close()
+        base = best[1]
         for items in (base // 2, base * 2):
             if items < 1024 or items % 64:
                 continue
-            plan = T2MPlan(ipix, nx, ny, tile=best.tile, item_samples=items)
+            plan = T2MPlan(ipix, nx, ny, tile=best[0], item_samples=items)
             ms = timed(plan)
-            results.append((best.tile, items, ms))
+            results.append((best[0], items, ms))
             if ms < best_ms:
-                best.close()
-                best, best_ms = plan, ms
-            else:
-                plan.close()
-        c = {"tile": list(best.tile), "item_samples": int(best.item_samples), "ms": best_ms,
+                best, best_ms = (best[0], items), ms
+            plan.close()
+        c = {"tile": list(best[0]), "item_samples": int(best[1]), "ms": best_ms,
              "tried": [[list(t), int(i), m] for t, i, m in results], "when": time.strftime("%Y-%m-%d %H:%M:%S")}
         t2m[sig] = c
         self.chosen["t2m"] = c
@@ -258,7 +255,7 @@ class Tuner:
         if self.verbose:
             for t, i, m in results:
                 print("tune: t2m tile %s items %d  %.3f ms" % (t, i, m))
-        return best
+        return T2MPlan(ipix, nx, ny, tile=best[0], item_samples=best[1])
 
     # -------------------------------------------------------------- generic
     def choose(self, kind, sig, candidates, time_fn):

@@ -89,6 +89,43 @@ def test_graph_replay_matches_eager():
         torch.cuda.synchronize()
 
 
+def test_short_memory_operator_matches_default():
+    """The operator as built when device memory is short (filter in groups of
+    TODs sharing one work buffer, map2tod gathered from the row-major map)
+    gives the default operator's A.x and rhs bit for bit, eagerly and from a
+    captured graph."""
+    import torch
+    import minkasi_amd as mk
+    from minkasi_amd.mapmaking.batch import BatchedDot
+    todvec, m, otods = build(ntod=5, ndet=16, n=8001)
+    op = BatchedDot(todvec, m, tiled_gather=True)
+    work = [t.noise.filt32.work_bytes for t in todvec.tods]
+    small = BatchedDot(todvec, m, tiled_gather=False, filter_work_cap=2 * max(work))
+    assert op.tiled and len(op.filts) == 1
+    assert not small.tiled and small.ipix_g is None and len(small.filts) >= 3
+    assert sorted(i for g in small.filter_groups for i in g) == list(range(5))
+    assert small.filter_work.numel() <= 2 * max(work)
+    x = torch.randn((m.nx, m.ny), device=m.map.device)
+    want = torch.zeros_like(x)
+    op.apply(x, want, accumulate=False)
+    got = torch.zeros_like(x)
+    small.apply(x, got, accumulate=False)
+    assert torch.equal(got, want)
+    r0, r1 = torch.zeros_like(x), torch.zeros_like(x)
+    op.rhs(r0, accumulate=False)
+    small.rhs(r1, accumulate=False)
+    assert torch.equal(r0, r1)
+    g = torch.zeros_like(x)
+    small.capture(x, g, accumulate=False)
+    for k in range(3):
+        g.fill_(7.0)
+        small.apply(x, g, accumulate=False)
+        torch.cuda.synchronize()
+        assert torch.equal(g, want), k
+    rep = small.memory_report()
+    assert rep["tiled_gather_ipix"] == 0 and rep["filter_groups"] == len(small.filts)
+
+
 def test_run_pcg_matches_oracle():
     import torch
     import minkasi_amd as mk
