@@ -3,7 +3,8 @@
 Public names mirror the reference package (minkasi/__init__.py,
 minkasi/minkasi_all.py): Tod, TodVec, SkyMap, PolMap, Mapset,
 NoiseSmoothedSVD, run_pcg, run_pcg_wprior, make_hits, Cuts, CutsCompact,
-tsModel, tsMultiModel, truncate_tod, downsample_tod, fit_cm_plus_poly.
+tsModel, tsMultiModel, truncate_tod, downsample_tod, fit_cm_plus_poly,
+gapfill_lin, read_tod_from_fits (and the other TOD readers).
 Device work runs in libmkgpu (HIP, gfx950); importing the compute modules
 raises if that library is missing.
 """
@@ -42,7 +43,11 @@ def __getattr__(name):
     if name in ("myrank", "nproc", "comm", "have_mpi", "MPI", "barrier", "report_mpi", "set_nthread", "get_nthread"):
         from . import parallel
         return getattr(parallel, name)
-    if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens"):
+    if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens", "gapfill_lin"):
         from .tods import processing
         return getattr(processing, name)
+    if name in ("read_tod_from_fits", "read_tod_from_fits_cbass", "read_tod_from_fits_NIKA2", "read_octave_struct",
+                "todvec_from_files_octave", "cut_blacklist", "fits_open"):
+        from .tods import io
+        return getattr(io, name)
     raise AttributeError(name)

@@ -160,3 +160,36 @@ def fit_cm_plus_poly(dat, ord=2, cm_ord=1, niter=1, medsub=False, full_out=False
     if full_out:
         return dd, pred2, cm
     return dd
+
+
+def gapfill_lin(dat):
+    """Linear interpolation over NaN runs (tods/processing.py:286-354), in
+    place on a host array ndet x nsamp.  Returns (dat, cuts) with cuts[det]
+    the [last good sample before, first good sample after] pair of each run.
+    As in the reference, a run that starts at sample 0 takes its left edge
+    from the last sample (numpy's index -1) and is left unfilled, a run that
+    reaches the last sample raises IndexError, and a detector that is NaN
+    throughout raises ValueError."""
+    dat = np.asarray(dat)
+    ndet, nsamp = dat.shape
+    samps = np.arange(nsamp)
+    full_cuts = {}
+    for i in range(ndet):
+        bad = np.flatnonzero(np.isnan(dat[i]))
+        if len(bad) == 0:
+            continue
+        if len(bad) == nsamp:
+            raise ValueError("Error: detector non-responsive. Please cut before passing to gapfill_lin.")
+        brk = np.flatnonzero(np.diff(bad) > 1)
+        starts = np.concatenate([[0], brk + 1])
+        ends = np.concatenate([brk, [len(bad) - 1]])
+        cur = []
+        for s0, e0 in zip(starts, ends):
+            lo, hi = samps[bad[s0] - 1], samps[bad[e0] + 1]
+            lo_v, hi_v = dat[i, bad[s0] - 1], dat[i, bad[e0] + 1]
+            cur.append([lo, hi])
+            miss = np.arange(lo + 1, hi)
+            dat[i, miss] = np.interp(miss, [lo, hi], [lo_v, hi_v])
+        full_cuts[i] = cur
+    return dat, full_cuts
+

@@ -81,8 +81,11 @@ def test_flat_namespace_and_single_rank_names():
     import minkasi_amd.minkasi_all as flat
     for name in ("Tod", "TodVec", "SkyMap", "PolMap", "Mapset", "NoiseSmoothedSVD", "run_pcg", "run_pcg_wprior",
                  "make_hits", "tsModel", "CutsCompact", "truncate_tod", "downsample_tod", "fit_cm_plus_poly",
-                 "fft_r2r", "myrank", "nproc", "barrier", "comm"):
+                 "fft_r2r", "myrank", "nproc", "barrier", "comm", "read_tod_from_fits", "read_tod_from_fits_cbass",
+                 "read_tod_from_fits_NIKA2", "gapfill_lin", "todvec_from_files_octave", "cut_blacklist"):
         assert hasattr(flat, name), name
+    for name in ("read_tod_from_fits", "read_tod_from_fits_cbass", "gapfill_lin", "cut_blacklist", "Tod", "run_pcg"):
+        assert getattr(mk, name) is getattr(flat, name), name
     assert (mk.myrank, mk.nproc, mk.have_mpi, mk.comm) == (0, 1, False, None)
     assert mk.get_nthread() >= 1 and mk.set_nthread(4) is None
     mk.barrier()  # no device work queued, no GPU needed
