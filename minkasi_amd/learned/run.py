@@ -46,6 +46,9 @@ def parse(argv=None):
     ap.add_argument("--calibrate", default="", help="K^T K scale factors tried on a held-out problem, e.g. "
                     "0.25,0.5,1,2 (empty: none)")
     ap.add_argument("--calib-ntod", type=int, default=64, help="TODs of the held-out calibration problem")
+    ap.add_argument("--calibrate-levels", action="store_true",
+                    help="levels/tent: then scale each level's K^T K term by coordinate search on the held-out "
+                         "problem")
     ap.add_argument("--eval", action="store_true")
     ap.add_argument("--eval-ntod", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -311,6 +314,58 @@ def calibrate(P, params, arch, spec, tol, maxiter, factors=(0.25, 0.5, 1.0, 2.0,
     return scale_ktk(params, arch, spec, best[0]), best[0], rows
 
 
+def scale_levels(params, arch, spec, factors):
+    """Parameters with level l's K^T K term scaled by factors[l] (levels and
+    tent: K^T K is a sum of one term per level)."""
+    from .nets import param_layout
+    if arch == "chain":
+        raise ValueError("per-level scales need the levels or tent architecture")
+    p = np.array(params, dtype=np.float32, copy=True)
+    lay, _ = param_layout(arch, len(spec))
+    for (off, co, ci), c in zip(lay, factors):
+        p[off:off + co * ci * 9] *= np.float32(np.sqrt(c))
+    return p
+
+
+def calibrate_levels(Ps, params, arch, spec, tol, maxiter, steps=((0.5, 2.0),), max_sweeps=3):
+    """Per-level K^T K scales by coordinate search on the held-out problems
+    Ps: each level's scale is multiplied by each factor of a step in turn
+    and kept only when PCG then needs strictly fewer iterations to tol,
+    summed over the problems (several problems and strict improvement keep
+    the four scales from fitting one problem's iteration counts); sweeps
+    repeat until one changes nothing.  Returns (params, factors, rows)."""
+    from ..workloads import solve_to_tol
+    from .precon import LearnedPrecon
+
+    def score(c):
+        pc = scale_levels(params, arch, spec, c)
+        tot = 0
+        for P in Ps:
+            res = solve_to_tol(P, LearnedPrecon(P.hits, pc, arch, spec), tol, maxiter)
+            tot += res["iters"] if res["reached"] else maxiter + 1
+        return tot
+
+    c = [1.0] * len(spec)
+    best = score(c)
+    rows = [(list(c), best)]
+    log("  levels %s -> %d iterations over %d problems" % (c, best, len(Ps)))
+    for fs in steps:
+        for _ in range(max_sweeps):
+            changed = False
+            for lvl in range(len(spec)):
+                for f in fs:
+                    t = list(c)
+                    t[lvl] *= f
+                    sc = score(t)
+                    rows.append((t, sc))
+                    if sc < best:
+                        best, c, changed = sc, t, True
+                        log("  levels %s -> %d iterations" % (["%.3g" % v for v in c], sc))
+            if not changed:
+                break
+    return scale_levels(params, arch, spec, c), c, rows
+
+
 def evaluate(P, params, arch, spec, tol, maxiter, jac=None):
     from ..workloads import map_distance, solve_to_tol
     from .precon import LearnedPrecon
@@ -345,8 +400,11 @@ def main(argv=None):
     if args.seeds == 0:
         if init is None:
             raise ValueError("--seeds 0 (no training) needs --init weights")
-        params, hist, meta = np.array(init, dtype=np.float32), [], dict(loss="none", arch=arch, spec=list(spec),
-                                                                         init=args.init)
+        params, hist = np.array(init, dtype=np.float32), []
+        meta = dict(load(args.init)[3]) if args.init.endswith(".npz") else dict(loss="none", arch=arch,
+                                                                                 spec=list(spec))
+        hist = meta.pop("history", [])
+        meta["init"] = args.init
     elif args.loss == "energy":
         params, hist, meta = train_energy(args.config, args.seeds, args.ntod, args.steps_per_seed, arch, spec,
                                           args.lr, args.seed, args.init_frac, args.max_iter, args.tol_final,
@@ -360,15 +418,24 @@ def main(argv=None):
         del ds
     meta.update(history=hist)
     torch.cuda.empty_cache()
-    if args.calibrate:
+    if args.calibrate or args.calibrate_levels:
         from ..synth import tod_shapes
-        factors = tuple(float(c) for c in args.calibrate.split(","))
         # held out from both the training problems and the benchmark problem
         Pc = build_problem(args.config, shapes=tod_shapes(args.config, args.calib_ntod, seed=29999),
                            seed_base=9_000_000)
-        params, cbest, rows = calibrate(Pc, params, arch, spec, args.tol, args.maxiter, factors)
-        meta.update(calibration=dict(factors=factors, iterations=[r[1] for r in rows], chosen=cbest))
-        log("calibrated K^T K scale: %g" % cbest)
+        if args.calibrate:
+            factors = tuple(float(c) for c in args.calibrate.split(","))
+            params, cbest, rows = calibrate(Pc, params, arch, spec, args.tol, args.maxiter, factors)
+            meta.update(calibration=dict(factors=factors, iterations=[r[1] for r in rows], chosen=cbest))
+            log("calibrated K^T K scale: %g" % cbest)
+        if args.calibrate_levels:
+            # a second held-out problem (other shapes, other TOD seeds)
+            Pc2 = build_problem(args.config, shapes=tod_shapes(args.config, args.calib_ntod, seed=29998),
+                                seed_base=9_100_000)
+            params, lf, rows = calibrate_levels([Pc, Pc2], params, arch, spec, args.tol, args.maxiter)
+            meta.update(level_calibration=dict(factors=lf, problems=2, tried=[[r[0], r[1]] for r in rows]))
+            log("calibrated per-level K^T K scales: %s" % lf)
+            del Pc2
         del Pc
         torch.cuda.empty_cache()
     res = dict(train=meta)

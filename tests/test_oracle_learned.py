@@ -234,3 +234,38 @@ def test_tent_levels_grad_finite_differences():
         lm = net.loss_and_grad(r, e, s, m)[0]
         net.ws[l][idx] = old
         assert abs((lp - lm) / (2 * h) - grads[l][idx]) < 1e-6 * max(1.0, abs(grads[l][idx]))
+
+
+def test_level_scales_scale_each_level_term():
+    """scale_levels (the per-level calibration of the learned preconditioner)
+    multiplies level l's K^T K term by c_l: with the oracle's tent-levels net
+    M(c) r = alpha S^2 r + sum_l c_l T_l r, each T_l measured as M(c = 2 at l,
+    1 elsewhere) r - M(1) r."""
+    from minkasi_amd.learned.nets import param_layout
+    from minkasi_amd.learned.run import scale_levels
+    rng = np.random.default_rng(21)
+    factors = (1, 2, 4)
+    lay, n = param_layout("tent", len(factors))
+    p = rng.standard_normal(n).astype(np.float32) * 0.3
+    c = [0.5, 3.0, 0.25]
+    q = scale_levels(p, "tent", factors, c)
+    assert q[-1] == p[-1]  # log alpha kept
+    for (off, co, ci), cl in zip(lay, c):
+        assert np.allclose(q[off:off + co * ci * 9], p[off:off + co * ci * 9] * np.sqrt(cl), rtol=1e-6)
+
+    def apply(params, r, s):
+        ws = [np.asarray(params[off:off + co * ci * 9], dtype=np.float64).reshape(co, ci, 9) for off, co, ci in lay]
+        return L.TentLevelsNet(ws, factors, log_alpha=float(params[-1])).apply(r, s)
+
+    r = rng.standard_normal((1, 1, 12, 11))
+    s = rng.uniform(0.3, 1.2, r.shape)
+    base = apply(p, r, s)
+    terms = [apply(scale_levels(p, "tent", factors, [2.0 if k == l else 1.0 for k in range(3)]), r, s) - base
+             for l in range(3)]
+    alpha_part = base - sum(terms)
+    assert np.allclose(alpha_part, np.exp(p[-1]) * s * s * r, rtol=0, atol=1e-5 * np.abs(base).max())
+    assert min(np.abs(t).max() for t in terms) > 1e-3 * np.abs(base).max()
+    want = alpha_part + sum(cl * t for cl, t in zip(c, terms))
+    assert np.allclose(apply(q, r, s), want, rtol=0, atol=1e-5 * np.abs(want).max())
+    with pytest.raises(ValueError):
+        scale_levels(p, "chain", factors, c)
