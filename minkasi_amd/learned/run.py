@@ -47,8 +47,10 @@ def parse(argv=None):
                     "0.25,0.5,1,2 (empty: none)")
     ap.add_argument("--calib-ntod", type=int, default=64, help="TODs of the held-out calibration problem")
     ap.add_argument("--calibrate-levels", action="store_true",
-                    help="levels/tent: then scale each level's K^T K term by coordinate search on the held-out "
-                         "problem")
+                    help="levels/tent: then scale each level's K^T K term by coordinate search on two held-out "
+                         "problems")
+    ap.add_argument("--level-steps", default="0.5,2",
+                    help="factor pairs of the per-level search, coarse to fine, e.g. 0.5,2;0.7,1.4")
     ap.add_argument("--eval", action="store_true")
     ap.add_argument("--eval-ntod", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -432,8 +434,14 @@ def main(argv=None):
             # a second held-out problem (other shapes, other TOD seeds)
             Pc2 = build_problem(args.config, shapes=tod_shapes(args.config, args.calib_ntod, seed=29998),
                                 seed_base=9_100_000)
-            params, lf, rows = calibrate_levels([Pc, Pc2], params, arch, spec, args.tol, args.maxiter)
-            meta.update(level_calibration=dict(factors=lf, problems=2, tried=[[r[0], r[1]] for r in rows]))
+            steps = tuple(tuple(float(v) for v in st.split(",")) for st in args.level_steps.split(";"))
+            params, lf, rows = calibrate_levels([Pc, Pc2], params, arch, spec, args.tol, args.maxiter, steps)
+            cal = dict(factors=lf, steps=steps, problems=2, tried=[[r[0], r[1]] for r in rows])
+            if "level_calibration" in meta:  # a further search on top of an earlier one
+                prev = meta["level_calibration"]
+                cal["after"] = prev
+                cal["factors_total"] = [a * b for a, b in zip(prev.get("factors_total", prev["factors"]), lf)]
+            meta.update(level_calibration=cal)
             log("calibrated per-level K^T K scales: %s" % lf)
             del Pc2
         del Pc
