@@ -73,7 +73,9 @@ int mk_pix_from_raster_car(const double *ra_b, const double *dec_b,
  * Replaces map2tod_omp / map2tod_simple (minkasi/lib/minkasi.c:116-144)
  * bound by minkasi/lib/minkasi.py (argtypes) and called from
  * minkasi/mapmaking/map2tod.py:19-54.  nsamp = ndet*ndata of one TOD or of
- * a whole batch of concatenated TODs.  Bit-exact (a gather). */
+ * a whole batch of concatenated TODs.  Bit-exact (a gather).  Any
+ * element-aligned ipix / dat: 16-byte aligned ipix with dat aligned to 4
+ * elements take 4-wide loads and stores, others one sample per thread. */
 int mk_map2tod_f32(float *dat, const float *map, int64_t nsamp,
                    const int32_t *ipix, int do_add, void *stream);
 int mk_map2tod_f64(double *dat, const double *map, int64_t nsamp,

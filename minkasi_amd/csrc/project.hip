@@ -87,16 +87,34 @@ __global__ void __launch_bounds__(256) tod2map_atomic_kernel(T* map, const T* __
     atomicAdd(map + ipix[i], dat[i]);
 }
 
+// One sample per thread, for ipix / dat that the 4-wide loads and stores of
+// map2tod_kernel cannot take (e.g. one TOD's view into a batched index
+// arena, which starts wherever the previous TODs end).
+template <typename T>
+__global__ void __launch_bounds__(256) map2tod_scalar_kernel(T* __restrict__ dat, const T* __restrict__ map,
+                                                             int64_t n, const int32_t* __restrict__ ipix,
+                                                             int do_add) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T v = map[ipix[i]];
+    dat[i] = do_add ? dat[i] + v : v;
+  }
+}
+
 template <typename T>
 int map2tod_impl(T* dat, const T* map, int64_t nsamp, const int32_t* ipix, int do_add, void* stream) {
   MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
   if (nsamp == 0) return MK_OK;
   MK_REQUIRE(dat && map && ipix, MK_ERR_ARG, "null pointer");
-  MK_REQUIRE(((uintptr_t)ipix & 15) == 0 && ((uintptr_t)dat % (4 * sizeof(T))) == 0, MK_ERR_ARG,
-             "map2tod: ipix must be 16-byte aligned and dat aligned to 4 elements");
-  map2tod_kernel<T><<<mk::grid_for(mk::cdiv(nsamp, 32), 256, 8192), 256, 0, mk::as_stream(stream)>>>(
-      dat, map, nsamp, ipix, do_add);
-  return mk::check_launch("map2tod_kernel");
+  MK_REQUIRE(((uintptr_t)ipix & 3) == 0 && ((uintptr_t)dat % sizeof(T)) == 0, MK_ERR_ARG,
+             "map2tod: ipix and dat must be aligned to their element size");
+  hipStream_t st = mk::as_stream(stream);
+  if (((uintptr_t)ipix & 15) == 0 && ((uintptr_t)dat % (4 * sizeof(T))) == 0) {
+    map2tod_kernel<T><<<mk::grid_for(mk::cdiv(nsamp, 32), 256, 8192), 256, 0, st>>>(dat, map, nsamp, ipix, do_add);
+    return mk::check_launch("map2tod_kernel");
+  }
+  map2tod_scalar_kernel<T><<<mk::grid_for(nsamp, 256), 256, 0, st>>>(dat, map, nsamp, ipix, do_add);
+  return mk::check_launch("map2tod_scalar_kernel");
 }
 
 // ---------------------------------------------------------------------------

@@ -112,6 +112,36 @@ def test_fits_tod_through_device_path():
     assert np.array_equal(hits.map.cpu().numpy().ravel(), want.astype(np.float32))
 
 
+def test_per_tod_path_after_batched_operator():
+    """Once the batched operator has made every TOD's ipix a view into its
+    index arena (offsets that are not multiples of 4 here), the per-TOD
+    path still works: TodVec.dot over FP64 copies of the TODs, which runs
+    map2tod TOD by TOD, matches the oracle's A.x."""
+    import torch
+    import minkasi_amd as mk
+    todvec, m, otods = build(ntod=3, ndet=7, n=4001)
+    ms = mk.Mapset()
+    ms.add_map(m)
+    op = todvec.batched(ms)
+    assert any(o % 4 for o in op.offs)
+    tv = mk.TodVec()
+    for tod in todvec.tods:
+        ip = tod.info["ipix"]
+        assert ip.data_ptr() != 0 and ip.is_contiguous()
+        d = tod.get_data()
+        d = d.to(torch.float64) if torch.is_tensor(d) else np.asarray(d, dtype=np.float64)
+        tv.add_tod(mk.Tod({"dat_calib": d, "ipix": ip}))
+    for t64, ot in zip(tv.tods, otods):
+        t64.set_noise(mk.NoiseSmoothedSVD, fwhm=10.0, u_in=ot.noise.v.T.copy(), keep_fp64=True)
+    m64 = m.copy()
+    m64.map = torch.randn(m.nx, m.ny, dtype=torch.float64, device=m.map.device)
+    x = mk.Mapset()
+    x.add_map(m64)
+    out = tv.dot(x)
+    ref = solve.todvec_dot(otods, m64.map.cpu().numpy())
+    assert rel_err(out.maps[0].map.cpu().numpy(), ref) < 1e-10
+
+
 def test_short_memory_operator_matches_default():
     """The operator as built when device memory is short (filter in groups of
     TODs sharing one work buffer, map2tod gathered from the row-major map)
