@@ -333,6 +333,15 @@ int64_t mk_gram_workspace_bytes(int64_t M, int64_t n);
 int mk_gram_f64(int64_t M, int64_t n, const void *D, int d_is_f32,
                 int64_t ldd, double *G, void *work, int64_t work_bytes,
                 void *stream);
+/* C = A.B^T (M x N, FP64) for row-major A (M x n) and B (N x n) with a long
+ * inner axis n, split over n-chunks and summed in chunk order: the
+ * np.dot(tmp, uu.T) cross products of gapfill_eig
+ * (minkasi/tods/processing.py:357-421).  work: mk_cross_workspace_bytes
+ * bytes. */
+int64_t mk_cross_workspace_bytes(int64_t M, int64_t N, int64_t n);
+int mk_cross_f64(int64_t M, int64_t N, int64_t n, const double *A,
+                 int64_t lda, const double *B, int64_t ldb, double *C,
+                 void *work, int64_t work_bytes, void *stream);
 /* Spectrum smoothing (minkasi/tools/smooth.py:49-78 smooth_many_vecs and
  * noise.py:674-678): out_k = sum_{t=t_lo..t_hi} g[|t|] P(k-t) over the even
  * periodic extension of each row (period 2(n-1)); P = x^2 if flags&1 else

@@ -43,7 +43,8 @@ def __getattr__(name):
     if name in ("myrank", "nproc", "comm", "have_mpi", "MPI", "barrier", "report_mpi", "set_nthread", "get_nthread"):
         from . import parallel
         return getattr(parallel, name)
-    if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens", "gapfill_lin"):
+    if name in ("truncate_tod", "downsample_tod", "fit_cm_plus_poly", "find_good_fft_lens", "gapfill_lin",
+                "gapfill_eig"):
         from .tods import processing
         return getattr(processing, name)
     if name in ("read_tod_from_fits", "read_tod_from_fits_cbass", "read_tod_from_fits_NIKA2", "read_octave_struct",

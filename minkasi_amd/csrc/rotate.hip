@@ -293,10 +293,13 @@ __global__ void __launch_bounds__(256) gemm_nn_f64(int M, int N, int K, const do
     }
 }
 
-// Partial Gram matrices: P[chunk][M x M] = D[:, chunk] . D[:, chunk]^T.
-template <typename TD>
-__global__ void __launch_bounds__(256) gram_partial_f64(int M, int64_t n, const TD* __restrict__ D, int64_t ldd,
-                                                        int64_t chunk, double* __restrict__ P) {
+// Partial cross products over chunks of the long axis:
+// P[chunk][M x N] = A[:, chunk] . B[:, chunk]^T (A: M rows, B: N rows, both
+// row-major with n columns; the Gram matrix is A = B).
+template <typename TA, typename TB2>
+__global__ void __launch_bounds__(256) cross_partial_f64(int M, int N, int64_t n, const TA* __restrict__ A,
+                                                         int64_t lda, const TB2* __restrict__ B, int64_t ldb,
+                                                         int64_t chunk, double* __restrict__ P) {
   __shared__ double As[TK][TB + 1];
   __shared__ double Bs[TK][TB + 1];
   const int bm = blockIdx.y * TB, bn = blockIdx.x * TB;
@@ -308,8 +311,8 @@ __global__ void __launch_bounds__(256) gram_partial_f64(int M, int64_t n, const 
       int mm = idx / TK, kk = idx % TK;  // consecutive threads walk along n
       int64_t gk = k0 + kk;
       int gm = bm + mm, gn = bn + mm;
-      As[kk][mm] = (gm < M && gk < c1) ? (double)D[(int64_t)gm * ldd + gk] : 0.0;
-      Bs[kk][mm] = (gn < M && gk < c1) ? (double)D[(int64_t)gn * ldd + gk] : 0.0;
+      As[kk][mm] = (gm < M && gk < c1) ? (double)A[(int64_t)gm * lda + gk] : 0.0;
+      Bs[kk][mm] = (gn < N && gk < c1) ? (double)B[(int64_t)gn * ldb + gk] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -326,13 +329,13 @@ __global__ void __launch_bounds__(256) gram_partial_f64(int M, int64_t n, const 
     }
     __syncthreads();
   }
-  double* out = P + (int64_t)blockIdx.z * M * M;
+  double* out = P + (int64_t)blockIdx.z * M * N;
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       int gm = bm + ty + 16 * i, gn = bn + tx + 16 * j;
-      if (gm < M && gn < M) out[(int64_t)gm * M + gn] = acc[i][j];
+      if (gm < M && gn < N) out[(int64_t)gm * N + gn] = acc[i][j];
     }
 }
 
@@ -476,12 +479,36 @@ extern "C" int mk_gram_f64(int64_t M, int64_t n, const void* D, int d_is_f32, in
   dim3 grid((unsigned)mk::cdiv(M, TB), (unsigned)mk::cdiv(M, TB), (unsigned)nchunk);
   hipStream_t st = mk::as_stream(stream);
   if (d_is_f32)
-    gram_partial_f64<float><<<grid, 256, 0, st>>>((int)M, n, (const float*)D, ldd, chunk, (double*)work);
+    cross_partial_f64<float, float><<<grid, 256, 0, st>>>((int)M, (int)M, n, (const float*)D, ldd, (const float*)D,
+                                                          ldd, chunk, (double*)work);
   else
-    gram_partial_f64<double><<<grid, 256, 0, st>>>((int)M, n, (const double*)D, ldd, chunk, (double*)work);
-  int r = mk::check_launch("gram_partial_f64");
+    cross_partial_f64<double, double><<<grid, 256, 0, st>>>((int)M, (int)M, n, (const double*)D, ldd,
+                                                            (const double*)D, ldd, chunk, (double*)work);
+  int r = mk::check_launch("cross_partial_f64");
   if (r) return r;
   sum_partials_f64<<<mk::grid_for(M * M, 256), 256, 0, st>>>((const double*)work, nchunk, M * M, G);
+  return mk::check_launch("sum_partials_f64");
+}
+
+extern "C" int64_t mk_cross_workspace_bytes(int64_t M, int64_t N, int64_t n) {
+  int64_t nchunk = std::max<int64_t>(1, mk::cdiv(n, 8192));
+  return nchunk * M * N * 8 + 256;
+}
+
+extern "C" int mk_cross_f64(int64_t M, int64_t N, int64_t n, const double* A, int64_t lda, const double* B,
+                            int64_t ldb, double* C, void* work, int64_t work_bytes, void* stream) {
+  MK_REQUIRE(M > 0 && N > 0 && n > 0 && A && B && C && work, MK_ERR_ARG, "bad arguments");
+  MK_REQUIRE(M < 65536 * TB && N < 65536 * TB && lda >= n && ldb >= n, MK_ERR_ARG, "bad shapes");
+  MK_REQUIRE(work_bytes >= mk_cross_workspace_bytes(M, N, n), MK_ERR_ARG, "workspace too small");
+  const int64_t chunk = 8192;
+  const int64_t nchunk = std::max<int64_t>(1, mk::cdiv(n, chunk));
+  MK_REQUIRE(nchunk < 65536, MK_ERR_ARG, "n too large");
+  dim3 grid((unsigned)mk::cdiv(N, TB), (unsigned)mk::cdiv(M, TB), (unsigned)nchunk);
+  hipStream_t st = mk::as_stream(stream);
+  cross_partial_f64<double, double><<<grid, 256, 0, st>>>((int)M, (int)N, n, A, lda, B, ldb, chunk, (double*)work);
+  int r = mk::check_launch("cross_partial_f64");
+  if (r) return r;
+  sum_partials_f64<<<mk::grid_for(M * N, 256), 256, 0, st>>>((const double*)work, nchunk, M * N, C);
   return mk::check_launch("sum_partials_f64");
 }
 

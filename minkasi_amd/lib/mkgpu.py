@@ -101,6 +101,8 @@ _SIGS = {
     "mk_gemm_nn_f64": ([I64, I64, I64, P, I64, P, I, I64, P, I64, P], I),
     "mk_gram_workspace_bytes": ([I64, I64], I64),
     "mk_gram_f64": ([I64, I64, P, I, I64, P, P, I64, P], I),
+    "mk_cross_workspace_bytes": ([I64, I64, I64], I64),
+    "mk_cross_f64": ([I64, I64, I64, P, I64, P, I64, P, P, I64, P], I),
     "mk_smooth_rows_f64": ([P, P, P, I64, I64, P, I32, I32, I32, P], I),
     "mk_median_cols_f32": ([P, I64, I64, I64, P, P], I),
     "mk_median_cols_f64": ([P, I64, I64, I64, P, P], I),

@@ -193,3 +193,74 @@ def gapfill_lin(dat):
         full_cuts[i] = cur
     return dat, full_cuts
 
+
+
+def _cross64(A, B):
+    """A . B^T on the device in FP64 (A: M x n, B: N x n, contiguous)."""
+    M, n = A.shape
+    N = B.shape[0]
+    C = torch.empty((M, N), dtype=torch.float64, device=A.device)
+    nb = int(mkgpu.lib.mk_cross_workspace_bytes(M, N, n))
+    work = torch.empty(nb // 8 + 32, dtype=torch.float64, device=A.device)
+    mkgpu.check("mk_cross_f64", M, N, n, dev.ptr(A, torch.float64, "A"), n, dev.ptr(B, torch.float64, "B"), n,
+                C.data_ptr(), work.data_ptr(), work.numel() * 8, dev.stream_ptr())
+    return C
+
+
+def gapfill_eig(dat, cuts, tod=None, thresh=5.0, niter_eig=3, niter_inner=3, insert_cuts=False):
+    """Fits the cut samples of dat (ndet x n) from its dominant detector
+    eigenmodes (tods/processing.py:357-421), returning a copy of the
+    CutsCompact `cuts` holding the fitted values.
+
+    niter_eig times: the data with the current fill at the cut samples gives
+    the detector covariance tmp.tmp^T (device, FP64); its eigenvectors with
+    eigenvalues above thresh^2 times the median (numpy eig on the host, as
+    the reference) span the modes uu = vv^T tmp; niter_inner times the data
+    are fitted by those modes and the fit replaces the cut samples.  The
+    long products (tmp.tmp^T, vv^T tmp, uu.uu^T, tmp.uu^T, fit.uu) are
+    device FP64 kernels; the eigen-decomposition and the neig x neig inverse
+    are host numpy.  With insert_cuts the fill is added into dat at the cut
+    samples (in place; dat must then be a device FP64 tensor or a numpy
+    array)."""
+    from ..mapmaking.noise import _gemm64
+    host = not isinstance(dat, torch.Tensor)
+    d = dev.to_device(np.ascontiguousarray(dat), torch.float64) if host else dat.to(torch.float64).contiguous()
+    if d.dim() != 2:
+        raise ValueError("dat must be ndet x n")
+    cuts_cur = cuts.copy()
+    if len(cuts_cur.imap) == 0 and int(np.sum(cuts_cur.nseg)) > 0:
+        cuts_cur.get_imap()
+    out_dtype = cuts_cur.map.dtype
+    cuts_cur.map = cuts_cur.map.to(torch.float64)
+    cuts_cur.clear()
+    for _ in range(niter_eig):
+        tmp = d.clone()
+        cuts_cur.map2tod(tod, tmp, do_add=False)
+        mycov = _cross64(tmp, tmp).cpu().numpy()
+        ee, vv = np.linalg.eig(mycov)
+        mask = ee > thresh * thresh * np.median(ee)
+        neig = int(np.sum(mask))
+        print("working with " + repr(neig) + " eigenvectors.")
+        if neig == 0:  # the fit of no modes is zero
+            cuts_cur.clear()
+            cuts_cur.map2tod(tod, tmp, do_add=False)
+            continue
+        vv = np.real(vv[:, mask])
+        uu = _gemm64(torch.from_numpy(np.ascontiguousarray(vv.T)).to(d.device), tmp)
+        lhs_inv = np.linalg.inv(_cross64(uu, uu).cpu().numpy())
+        for __ in range(niter_inner):
+            rhs = _cross64(tmp, uu).cpu().numpy()                       # ndet x neig
+            fitp = lhs_inv @ rhs.T                                       # neig x ndet
+            pred = _gemm64(torch.from_numpy(np.ascontiguousarray(fitp.T)).to(d.device), uu)
+            cuts_cur.tod2map(tod, pred, do_add=False)
+            cuts_cur.map2tod(tod, tmp, do_add=False)
+    if insert_cuts:
+        if host:
+            cuts_cur.map2tod(tod, d)
+            np.copyto(dat, d.cpu().numpy().reshape(np.shape(dat)).astype(np.asarray(dat).dtype, copy=False))
+        else:
+            if dat.dtype != torch.float64 or not dat.is_contiguous():
+                raise TypeError("insert_cuts updates dat in place: pass a contiguous float64 device tensor")
+            cuts_cur.map2tod(tod, dat)
+    cuts_cur.map = cuts_cur.map.to(out_dtype)
+    return cuts_cur

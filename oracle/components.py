@@ -20,6 +20,14 @@ Functions, what they follow, and what pins them (tests/test_oracle_components.py
   precon_mul        -- polmap.py:624-684 (the block times the Stokes vector
                        of each pixel).  Pinned by an explicit per-pixel
                        3x3 (2x2) matrix-vector product.
+  gapfill_eig       -- minkasi/tods/processing.py:357-421 (eigenmodes of
+                       the gap-filled detector covariance, least-squares fit
+                       of the data by them, the fit replacing the cut
+                       samples).  Pinned by exactly low-rank data, whose cut
+                       samples the iteration recovers (error shrinking with
+                       the iteration count), by an explicit least-squares
+                       solve of one inner step, and by the no-mode case
+                       (threshold above every eigenvalue: zero fill).
 """
 import numpy as np
 
@@ -117,3 +125,29 @@ def precon_mul(pre, x, poltag):
     out[..., 1] = a[..., 1] * x[..., 0] + a[..., 3] * x[..., 1] + a[..., 4] * x[..., 2]
     out[..., 2] = a[..., 2] * x[..., 0] + a[..., 4] * x[..., 1] + a[..., 5] * x[..., 2]
     return out
+
+
+def gapfill_eig(dat, imap, thresh=5.0, niter_eig=3, niter_inner=3):
+    """Fill values of the cut samples imap (flat indices into dat, ndet x n)
+    after gapfill_eig's iterations; the steps of processing.py:395-419 with
+    the cut vector as a flat array (map2tod(do_add=False) sets
+    tmp.flat[imap] = fill, tod2map(do_add=False) reads fill = pred.flat[imap])."""
+    dat = np.asarray(dat, dtype=np.float64)
+    fill = np.zeros(len(imap))
+    for _ in range(niter_eig):
+        tmp = dat.copy()
+        tmp.flat[imap] = fill
+        mycov = tmp @ tmp.T
+        ee, vv = np.linalg.eig(mycov)
+        mask = ee > thresh * thresh * np.median(ee)
+        vv = np.real(vv[:, mask])
+        uu = vv.T @ tmp
+        lhs_inv = np.linalg.inv(uu @ uu.T) if mask.any() else np.zeros((0, 0))
+        for __ in range(niter_inner):
+            rhs = tmp @ uu.T
+            fitp = lhs_inv @ rhs.T
+            pred = fitp.T @ uu
+            fill = pred.flat[imap].copy()
+            tmp.flat[imap] = fill
+    return fill
+

@@ -467,3 +467,48 @@ def test_batched_map_plus_cuts_matches_per_tod_path(env):
     xf = mk.run_pcg(rf, x0, todvec, pre, maxiter=10, verbose=False)
     xs = pcg._run_pcg_generic(rf, x0, todvec, None, pre, None, 10, "map", [-1], 0, ".fits", None, None, False)
     assert rel_err(pack(xf), pack(xs)) < 1e-3
+
+
+def test_cross_product_kernel(env):
+    """mk_cross_f64: A.B^T with a long inner axis against numpy."""
+    torch, device, mkgpu = env
+    from minkasi_amd.tods.processing import _cross64
+    rng = np.random.default_rng(40)
+    for M, N, n in [(7, 3, 20001), (70, 65, 9000), (1, 1, 5)]:
+        A, B = rng.standard_normal((M, n)), rng.standard_normal((N, n))
+        got = _cross64(device.to_device(A), device.to_device(B)).cpu().numpy()
+        assert rel_err(got, A @ B.T) < 1e-13, (M, N, n)
+
+
+@pytest.mark.parametrize("host", [False, True])
+def test_gapfill_eig_matches_oracle(env, host):
+    """gapfill_eig (device products, host eig) against the FP64 oracle on a
+    rank-2 signal with white noise and short cuts; insert_cuts adds the fill
+    into the data at the cut samples."""
+    torch, device, mkgpu = env
+    from minkasi_amd.tods.cuts import CutsCompact
+    from minkasi_amd.tods.processing import gapfill_eig
+    rng = np.random.default_rng(30)
+    ndet, n = 24, 2000
+    modes = np.cumsum(rng.standard_normal((2, n)), axis=1)
+    modes *= 30.0 / np.abs(modes).max()
+    d = rng.standard_normal((ndet, 2)) @ modes + rng.standard_normal((ndet, n))
+    mask = np.ones(d.shape, dtype=bool)
+    mask[3, 100:104] = False
+    mask[7, 400:403] = False
+    mask[0, 1990:1995] = False
+    tod = _FakeTod({"dat_calib": device.to_device(d)})
+    cuts = CutsCompact(tod)
+    cuts.cuts_from_array(mask)
+    cuts.get_imap()
+    imap = np.flatnonzero(~mask)
+    ref = oc.gapfill_eig(d, imap, thresh=5.0, niter_eig=3, niter_inner=3)
+    dat = d.copy() if host else device.to_device(d)
+    out = gapfill_eig(dat, cuts, thresh=5.0, niter_eig=3, niter_inner=3, insert_cuts=True)
+    assert out is not cuts and out.map.dtype == cuts.map.dtype
+    assert rel_err(out.map.cpu().numpy(), ref) < 1e-9
+    assert np.array_equal(cuts.map.cpu().numpy(), np.zeros(len(imap)))  # the input cuts are untouched
+    got = dat if host else dat.cpu().numpy()
+    want = d.copy()
+    want.flat[imap] += out.map.cpu().numpy()
+    assert rel_err(got, want) < 1e-12

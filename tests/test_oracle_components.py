@@ -199,3 +199,51 @@ def test_precon_mul_is_blockwise_matvec(poltag, k):
     x = rng.standard_normal((30, 1, k))
     out = oc.precon_mul(_pack(B, poltag), x, poltag)
     assert np.allclose(out.reshape(30, k), np.einsum("nij,nj->ni", B, x.reshape(30, k)), rtol=1e-13, atol=1e-13)
+
+
+def _lowrank(rng, ndet=12, n=600, k=2, amp=None):
+    modes = np.cumsum(rng.standard_normal((k, n)), axis=1)
+    if amp is not None:
+        modes *= amp / np.abs(modes).max()
+    mix = rng.standard_normal((ndet, k))
+    return mix @ modes
+
+
+def test_gapfill_eig_recovers_low_rank_signal():
+    """Rank-2 signal (amplitude 30) plus unit white noise over 24 detectors,
+    with short cuts: the fill recovers the signal at the cut samples to the
+    noise level, better after more iterations than after one."""
+    rng = np.random.default_rng(30)
+    sig = _lowrank(rng, ndet=24, n=2000, amp=30.0)
+    d = sig + rng.standard_normal(sig.shape)
+    mask = np.ones(d.shape, dtype=bool)
+    mask[3, 100:104] = False
+    mask[7, 400:403] = False
+    mask[0, 1990:1995] = False
+    imap = np.flatnonzero(~mask)
+    truth = sig.flat[imap]
+    errs = [np.abs(oc.gapfill_eig(d, imap, niter_eig=ne, niter_inner=ni) - truth).max()
+            for ne, ni in [(1, 1), (3, 3), (10, 10)]]
+    assert errs[0] > 2 * errs[1] and max(errs[1:]) < 1.0 < np.abs(truth).max() / 10, errs
+
+
+def test_gapfill_eig_inner_step_is_least_squares():
+    """One eigen step with one inner step: the fill equals the cut samples of
+    the least-squares fit of the (zero-filled) data by the selected modes."""
+    rng = np.random.default_rng(31)
+    d = _lowrank(rng, k=3) + 0.01 * rng.standard_normal((12, 600))
+    imap = np.arange(5 * 600 + 50, 5 * 600 + 90)
+    fill = oc.gapfill_eig(d, imap, thresh=2.0, niter_eig=1, niter_inner=1)
+    tmp = d.copy()
+    tmp.flat[imap] = 0.0
+    w, v = np.linalg.eigh(tmp @ tmp.T)
+    uu = v[:, w > 4.0 * np.median(w)].T @ tmp
+    coef, *_ = np.linalg.lstsq(uu.T, tmp.T, rcond=None)  # tmp^T ~ uu^T coef
+    assert np.allclose(fill, (coef.T @ uu).flat[imap], rtol=1e-9, atol=1e-9 * np.abs(tmp).max())
+
+
+def test_gapfill_eig_without_modes_fills_zero():
+    rng = np.random.default_rng(32)
+    d = rng.standard_normal((6, 100))
+    imap = np.arange(10, 30)
+    assert np.array_equal(oc.gapfill_eig(d, imap, thresh=1e6), np.zeros(20))
