@@ -231,3 +231,39 @@ def test_energy_forward_backward_vs_oracle(env, arch, spec):
     ref = np.concatenate([gr.ravel() for gr in grads])
     assert rel(g[:-1], ref) < 1e-4
     assert abs(np.exp(net.log_alpha) * tr.dalpha() - dla) < 1e-4 * abs(dla)
+
+
+def test_shipped_weights_gain_over_jacobi(env):
+    """What the trained network gains over the method without it: the shipped
+    weights (minkasi_amd/learned/weights_c4.npz), on a 16-TOD problem of the
+    benchmark's distribution with seeds used neither in training nor in
+    calibration, reach ||r||/||b|| < 1e-6 in fewer PCG iterations than the
+    hits-count Jacobi preconditioner alone, to the same solution.  The
+    preconditioner they define is symmetric and positive on hit pixels."""
+    torch, device, mkgpu = env
+    import os
+    from minkasi_amd.learned.precon import LearnedPrecon, load
+    from minkasi_amd.synth import tod_shapes
+    from minkasi_amd.workloads import build_problem, map_distance, solve_to_tol
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "minkasi_amd", "learned",
+                        "weights_c4.npz")
+    params, arch, spec, meta = load(path)
+    P = build_problem(4, shapes=tod_shapes(4, 16, seed=777), seed_base=5_000_000)
+    lp = LearnedPrecon(P.hits, params, arch, spec)
+    hit = P.hits.map > 0
+    g = torch.Generator(device=P.hits.map.device).manual_seed(3)
+    x = torch.randn(P.hits.map.shape, device=P.hits.map.device, generator=g) * hit
+    y = torch.randn(P.hits.map.shape, device=P.hits.map.device, generator=g) * hit
+    mx, my = torch.empty_like(x), torch.empty_like(y)
+    lp.apply_to(x, mx)
+    lp.apply_to(y, my)
+    a, b = float((y.double() * mx.double()).sum()), float((x.double() * my.double()).sum())
+    assert abs(a - b) < 1e-4 * abs(a) and float((x.double() * mx.double()).sum()) > 0
+    jac = solve_to_tol(P, None, 1e-6, 400)
+    lrn = solve_to_tol(P, lp, 1e-6, 400)
+    print("16-TOD held-out problem: Jacobi %d its %.2f s, learned %d its %.2f s" % (
+        jac["iters"], jac["seconds"], lrn["iters"], lrn["seconds"]))
+    assert jac["reached"] and lrn["reached"]
+    assert lrn["iters"] <= 0.75 * jac["iters"], (lrn["iters"], jac["iters"])
+    assert lrn["true_rel_res"] < 3e-6
+    assert map_distance(P, lrn["x"], jac["x"]) < 1e-3
