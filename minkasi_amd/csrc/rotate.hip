@@ -172,6 +172,11 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
+  // A wave's 32-row sub-blocks that lie wholly past the segment's last row
+  // (the padding of M to the 128-row tile) skip their MFMAs; the condition
+  // is uniform over the wave and nothing of those rows is stored.
+  const bool live0 = m0 + wm * 64 < sg.M, live1 = m0 + wm * 64 + 32 < sg.M;
+
   load_stage(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
     store_stage();
@@ -195,6 +200,7 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
       for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
+          if (!(i == 0 ? live0 : live1)) continue;
           if (NP == 6) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
