@@ -133,6 +133,36 @@ def measured_traffic(stage):
     return None
 
 
+def copy_bandwidth(nbytes=1 << 31, reps=5):
+    """Achievable HBM bandwidth on this box, GB/s of bytes read + written:
+    the better of a device-to-device copy of nbytes (torch copy_) and the
+    library's streaming elementwise product o = a * b (mk_mul_f32, 12 bytes
+    per element), each event-timed over reps launches."""
+    import torch
+    from minkasi_amd import device as mdev
+    from minkasi_amd.lib import mkgpu
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    t_copy = timed(lambda: b.copy_(a))
+    t_mul = timed(lambda: mkgpu.check("mk_mul_f32", c.data_ptr(), a.data_ptr(), b.data_ptr(), n, mdev.stream_ptr()))
+    del a, b, c
+    torch.cuda.empty_cache()
+    return max(2.0 * nbytes / t_copy, 3.0 * nbytes / t_mul) / 1e9
+
+
 def copy_probe(nbytes):
     import torch
     n = nbytes // 4
@@ -250,7 +280,13 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom) if profiled else None,
             "algorithmic_bytes": int(sb[dom]), "stage_ms": {k: round(v, 3) for k, v in st.items()},
             "stage_frac": {k: round(sb[k] / (st[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k in st}}
+    copy_gbs = copy_bandwidth()
+    roof["copy_gbs"] = round(copy_gbs, 1)
+    roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    roof["stage_frac_of_copy"] = {k: round(sb[k] / (st[k] * 1e-3) / 1e9 / copy_gbs, 4) for k in st}
     log("stage ms:", json.dumps(roof["stage_ms"]), "frac:", json.dumps(roof["stage_frac"]))
+    log("device copy: %.0f GB/s (%.2f of the 8 TB/s peak); stage frac of copy: %s" % (
+        copy_gbs, copy_gbs / HBM_PEAK_GBS, json.dumps(roof["stage_frac_of_copy"])))
 
     result = {
         "metric": METRIC, "value": round(iters_per_s, 4), "unit": "PCG iter/s", "n_gpus": 1,
