@@ -141,6 +141,9 @@ def copy_bandwidth(nbytes=1 << 31, reps=5):
     import torch
     from minkasi_amd import device as mdev
     from minkasi_amd.lib import mkgpu
+    torch.cuda.empty_cache()
+    free = int(torch.cuda.mem_get_info()[0])
+    nbytes = min(nbytes, max(1 << 24, (free // 4) & ~((1 << 20) - 1)))  # three buffers within the free memory
     n = nbytes // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
