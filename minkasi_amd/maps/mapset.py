@@ -58,6 +58,19 @@ class Mapset(_MapsetBase):
             mm.maps[i] = self.maps[i] * mapset.maps[i]
         return mm
 
+    def apply_prior(self, x, Ax):
+        """mapset.py:248-275: each map's apply_prior, with the whole Mapset
+        for maps whose prior is global (isglobal_prior) and map by map
+        otherwise."""
+        for i in range(self.nmap):
+            m = self.maps[i]
+            if m is None:
+                continue
+            if getattr(m, "isglobal_prior", False):
+                m.apply_prior(x, Ax)
+            else:
+                m.apply_prior(x.maps[i], Ax.maps[i])
+
     def get_caches(self):
         for i in range(self.nmap):
             self.maps[i].get_caches()

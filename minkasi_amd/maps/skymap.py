@@ -109,6 +109,37 @@ class SkyMap:
     def clear_caches(self):
         self.caches = None
 
+    # The reference's per-method tod2map wrappers (skymap.py:310-397, 611-640).
+    # Its cached / everyone variants reduce per-thread map copies; here every
+    # one of them accumulates into self.map through the deterministic plan,
+    # except tod2map_atomic, which uses the global-atomic scatter.
+    def tod2map_atomic(self, tod, dat):
+        """skymap.py:310-323 (atomic adds, order-dependent in FP32)."""
+        from ..mapmaking.tod2map import tod2map_omp
+        tod2map_omp(self.map, dev.to_device(dat, self.map.dtype), self.get_pix(tod), atomic=True)
+
+    def tod2map_omp(self, tod, dat):
+        """skymap.py:325-338."""
+        self._t2m_plan(tod, self.get_pix(tod)).exec(self.map, dev.to_device(dat, self.map.dtype), True)
+
+    def tod2map_simple(self, tod, dat):
+        """skymap.py:340-353."""
+        self.tod2map_omp(tod, dat)
+
+    def tod2map_cached(self, tod, dat):
+        """skymap.py:379-397 (no per-thread caches are needed)."""
+        self.tod2map_omp(tod, dat)
+
+    def tod2map_everyone(self, tod, dat):
+        """skymap.py:355-377: the reference needs tod.info[tag + '_edges'],
+        the per-thread pixel ranges of its cached reduction; the plan needs
+        none, so they are not required."""
+        self.tod2map_omp(tod, dat)
+
+    def tod2map_old(self, tod, dat=None, do_add=True, do_omp=True):
+        """skymap.py:611-640."""
+        self.tod2map(tod, dat, do_add, do_omp)
+
     def set_tod2map(self, method=None, todvec=None):
         """skymap.py:254-308: stored, and honoured: 'atomic' selects the
         non-deterministic global-atomic scatter, anything else the plan."""

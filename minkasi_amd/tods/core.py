@@ -105,6 +105,48 @@ class Tod:
         f = torch.zeros if clear else torch.empty
         return f((self.get_ndet(), self.get_ndata()), dtype=dtype, device=dev.device())
 
+    def set_jumps(self, jumps):
+        """core.py:693-703."""
+        self.jumps = jumps
+
+    def cut_detectors(self, isgood):
+        """Keeps only the detectors where isgood is True (core.py:705-728):
+        every numpy array or device tensor in info whose first axis has one
+        entry per detector is sliced (a copy), and the per-detector jump and
+        cut lists drop the removed detectors.  Arrays with another first
+        axis are left alone, where the reference would index them by
+        detector number too.  The noise model is not rebuilt."""
+        isgood = np.asarray(isgood.cpu() if isinstance(isgood, torch.Tensor) else isgood).astype(bool)
+        ndet = self.get_ndet()
+        if isgood.shape != (ndet,):
+            raise ValueError("isgood has %s entries, the TOD %d detectors" % (isgood.shape, ndet))
+        keep = np.flatnonzero(isgood)
+        bad = np.flatnonzero(~isgood)[::-1]  # deleted in reverse, as the reference
+        for key in list(self.info.keys()):
+            v = self.info[key]
+            if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == ndet:
+                self.info[key] = np.take(v, keep, axis=0).copy()
+            elif isinstance(v, torch.Tensor) and v.dim() >= 1 and v.shape[0] == ndet:
+                self.info[key] = v.index_select(0, torch.from_numpy(keep).to(v.device)).contiguous()
+        if self.jumps is not None:
+            for i in bad:
+                del self.jumps[i]
+        if self.cuts is not None:
+            for i in bad:
+                del self.cuts[i]
+
+    def set_apix(self):
+        """info['apix'] (core.py:102-116): the detectors' elevation offsets
+        from a quadratic fit to the mean elevation track, scaled to +-1."""
+        elev = self.info["elev"]
+        elev = elev.detach().cpu().numpy() if isinstance(elev, torch.Tensor) else np.asarray(elev)
+        track_mean = np.mean(elev, axis=0)
+        x = np.arange(track_mean.shape[0]) / track_mean.shape[0]
+        a = np.polyfit(x, track_mean, 2)
+        track = a[2] + a[1] * x + a[0] * x ** 2
+        delev = elev - track[None, :]
+        self.info["apix"] = delev / np.max(np.abs(delev))
+
     def copy(self, copy_info=False):
         if copy_info:
             myinfo = self.info.copy()

@@ -89,6 +89,31 @@ def test_graph_replay_matches_eager():
         torch.cuda.synchronize()
 
 
+def test_skymap_tod2map_variants_agree():
+    """The reference's per-method wrappers (tod2map_omp / simple / cached /
+    everyone / old / atomic) all add P^T d into the map: bit-equal through
+    the deterministic plan, the atomic one to FP32 rounding; against the
+    oracle's tod2map."""
+    import torch
+    todvec, m, otods = build(ntod=2, ndet=8, n=3001)
+    tod, ot = todvec.tods[0], otods[0]
+    d = np.random.default_rng(2).standard_normal(ot.dat.shape).astype(np.float32)
+    ref = np.zeros(m.nx * m.ny)
+    core.tod2map(ref.reshape(1, -1), d.astype(np.float64), ot.ipix)
+    outs = {}
+    for name in ("tod2map_omp", "tod2map_simple", "tod2map_cached", "tod2map_everyone", "tod2map_atomic"):
+        m.clear()
+        getattr(m, name)(tod, d)
+        outs[name] = m.map.cpu().numpy().ravel().copy()
+    m.clear()
+    m.tod2map_old(tod, d)
+    outs["tod2map_old"] = m.map.cpu().numpy().ravel().copy()
+    for name, o in outs.items():
+        assert rel_err(o, ref) < 1e-6, name
+        if name != "tod2map_atomic":
+            assert np.array_equal(o, outs["tod2map_omp"]), name
+
+
 def test_fits_tod_through_device_path():
     """A MUSTANG TOD read from FITS (an astropy-written fixture) pixelizes on
     the device exactly as the oracle does, and its hit map counts every

@@ -74,3 +74,30 @@ def test_truncate_tod_cuts_every_sample_axis():
     assert np.array_equal(info["dec_b"].numpy(), ref["dec_b"].numpy()[:new])
     assert np.array_equal(info["dra"], ref["dra"]) and info["dt"] == 0.01
     assert truncate_tod({"ra_b": np.zeros(3)}) is None
+
+
+def test_tod_cut_detectors_set_jumps_and_apix():
+    """Tod.cut_detectors keeps the good detectors' rows of every
+    per-detector array and their jump lists (core.py:705-728); set_apix is
+    the elevation offset from a quadratic track fit, scaled to +-1
+    (core.py:102-116)."""
+    import minkasi_amd as mk
+    rng = np.random.default_rng(9)
+    ndet, n = 5, 40
+    x = np.arange(n) / n
+    track = 40 + 3 * x - 2 * x ** 2
+    off = rng.normal(0, 0.01, ndet)
+    info = {"dat_calib": rng.standard_normal((ndet, n)), "dx": rng.standard_normal((ndet, n)),
+            "pixid": np.arange(ndet) + 10, "mask_sum": np.arange(n), "elev": track[None, :] + off[:, None],
+            "fname": "a.fits"}
+    tod = mk.Tod(info)
+    tod.set_apix()
+    d = off - off.mean()  # the fit follows the mean track; each row is offset by a constant
+    assert np.allclose(tod.info["apix"], np.repeat((d / np.abs(d).max())[:, None], n, 1), atol=1e-9)
+    tod.set_jumps([[1], [], [3, 4], [], [7]])
+    good = np.array([True, False, True, True, False])
+    dat0 = info["dat_calib"].copy()
+    tod.cut_detectors(good)
+    assert np.array_equal(tod.info["dat_calib"], dat0[good]) and np.array_equal(tod.info["pixid"], [10, 12, 13])
+    assert np.array_equal(tod.info["mask_sum"], np.arange(n))  # not per detector: left alone
+    assert tod.jumps == [[1], [3, 4], []] and tod.get_ndet() == 3
