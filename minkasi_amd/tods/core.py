@@ -105,6 +105,38 @@ class Tod:
         f = torch.zeros if clear else torch.empty
         return f((self.get_ndet(), self.get_ndata()), dtype=dtype, device=dev.device())
 
+    def set_noise_smoothed_svd(self, fwhm=50, func=None, pars=None, prewhiten=False, fit_powlaw=False):
+        """The deprecated spelling of set_noise(NoiseSmoothedSVD)
+        (core.py:503-515 -> _depracated.py:77-93): the model is fitted to
+        dat_calib, or to dat_calib - func(pars, tod) when func is given."""
+        dat = self.get_data()
+        if func is not None:
+            model = func(pars, self)
+            dat = dat - (dev.to_device(model, dat.dtype) if isinstance(dat, torch.Tensor) else model)
+        self.set_noise(NoiseSmoothedSVD, dat, fwhm=fwhm, prewhiten=prewhiten, fit_powlaw=fit_powlaw)
+
+    def set_noise_white_masked(self):
+        """core.py:411-416: white noise of unit weight per detector."""
+        self.info["noise"] = "white_masked"
+        self.info["mywt"] = np.ones(self.get_ndet())
+
+    def apply_noise_white_masked(self, dat=None):
+        """mask * dat * mywt per detector (core.py:418-445), on the device
+        (elementwise products in the library); a new tensor of dat's dtype."""
+        from ..lib import mkgpu
+        d = dev.to_device(self.get_data(dat))
+        if d.dtype not in (torch.float32, torch.float64):
+            d = d.to(torch.float64)
+        d = d.contiguous()
+        sfx = "f32" if d.dtype == torch.float32 else "f64"
+        w = dev.to_device(np.asarray(self.info["mywt"], dtype=np.float64), d.dtype).reshape(-1, 1)
+        w = w.expand(d.shape).contiguous()
+        m = dev.to_device(self.info["mask"], d.dtype).expand(d.shape).contiguous()
+        out = torch.empty_like(d)
+        mkgpu.check("mk_mul_" + sfx, out.data_ptr(), m.data_ptr(), d.data_ptr(), d.numel(), dev.stream_ptr())
+        mkgpu.check("mk_mul_" + sfx, out.data_ptr(), out.data_ptr(), w.data_ptr(), d.numel(), dev.stream_ptr())
+        return out
+
     def set_jumps(self, jumps):
         """core.py:693-703."""
         self.jumps = jumps

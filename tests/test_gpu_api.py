@@ -114,6 +114,33 @@ def test_skymap_tod2map_variants_agree():
             assert np.array_equal(o, outs["tod2map_omp"]), name
 
 
+def test_tod_noise_wrappers():
+    """set_noise_smoothed_svd is set_noise(NoiseSmoothedSVD) (with func, of
+    the data minus the model); apply_noise_white_masked is mask * dat *
+    mywt per detector."""
+    import torch
+    import minkasi_amd as mk
+    rng = np.random.default_rng(6)
+    ndet, n = 6, 2001
+    dat = np.cumsum(rng.standard_normal((ndet, n)), axis=1)
+    a, b = mk.Tod({"dat_calib": dat.copy()}), mk.Tod({"dat_calib": dat.copy()})
+    a.set_noise_smoothed_svd(fwhm=20)
+    b.set_noise(mk.NoiseSmoothedSVD, fwhm=20)
+    x = rng.standard_normal((ndet, n))
+    assert rel_err(a.apply_noise(x).cpu().numpy(), b.apply_noise(x).cpu().numpy()) < 1e-12
+    model = np.outer(np.ones(ndet), np.linspace(0, 1, n))
+    c, e = mk.Tod({"dat_calib": dat.copy()}), mk.Tod({"dat_calib": dat - model})
+    c.set_noise_smoothed_svd(fwhm=20, func=lambda pars, tod: pars * model, pars=1.0)
+    e.set_noise(mk.NoiseSmoothedSVD, fwhm=20)
+    assert rel_err(c.apply_noise(x).cpu().numpy(), e.apply_noise(x).cpu().numpy()) < 1e-12
+    mask = rng.random((ndet, n)) > 0.1
+    w = mk.Tod({"dat_calib": dat.copy(), "mask": mask})
+    w.set_noise_white_masked()
+    w.info["mywt"] = np.arange(1, ndet + 1, dtype=float)
+    got = w.apply_noise_white_masked().cpu().numpy()
+    assert np.array_equal(got, mask * dat * w.info["mywt"][:, None])
+
+
 def test_fits_tod_through_device_path():
     """A MUSTANG TOD read from FITS (an astropy-written fixture) pixelizes on
     the device exactly as the oracle does, and its hit map counts every
