@@ -143,6 +143,22 @@ __global__ void __launch_bounds__(kBlock) mul_kernel(T* __restrict__ o, const T*
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) o[i] = a[i] * b[i];
 }
 
+// FP32 product with 16-byte loads and stores (all three arrays 16-byte
+// aligned); elements past the last full group of 4 are done one by one.
+__global__ void __launch_bounds__(kBlock) mul4_kernel(float* __restrict__ o, const float* __restrict__ a,
+                                                      const float* __restrict__ b, int64_t n) {
+  const int64_t n4 = n >> 2;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const float4* a4 = reinterpret_cast<const float4*>(a);
+  const float4* b4 = reinterpret_cast<const float4*>(b);
+  float4* o4 = reinterpret_cast<float4*>(o);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 x = a4[i], y = b4[i];
+    o4[i] = make_float4(x.x * y.x, x.y * y.y, x.z * y.z, x.w * y.w);
+  }
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) o[i] = a[i] * b[i];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) addmul_kernel(T* __restrict__ y, const T* __restrict__ a,
                                                         const T* __restrict__ b, int64_t n) {
@@ -165,6 +181,12 @@ template <typename T>
 int mul_impl(T* out, const T* a, const T* b, int64_t n, void* stream) {
   MK_REQUIRE(n >= 0 && (n == 0 || (out && a && b)), MK_ERR_ARG, "bad arguments");
   if (n == 0) return MK_OK;
+  if constexpr (sizeof(T) == 4) {
+    if ((((uintptr_t)out | (uintptr_t)a | (uintptr_t)b) & 15) == 0) {
+      mul4_kernel<<<mk::grid_for(mk::cdiv(n, 4), kBlock, 8192), kBlock, 0, mk::as_stream(stream)>>>(out, a, b, n);
+      return mk::check_launch("mul4_kernel");
+    }
+  }
   mul_kernel<T><<<mk::grid_for(n, kBlock), kBlock, 0, mk::as_stream(stream)>>>(out, a, b, n);
   return mk::check_launch("mul_kernel");
 }
