@@ -188,7 +188,7 @@ class BatchedDot:
         self._gx = self._gout = None
 
     @staticmethod
-    def _memory_plan(work, nsamp, tiled_gather, filter_work_cap):
+    def _memory_plan(work, nsamp, tiled_gather, filter_work_cap, free=None):
         """(tiled gather?, filter work cap in bytes) from the device memory
         free once the tod2map plan is built: less a reserve, it goes to the
         tiled gather's index copy (4 bytes per sample) and the filter work
@@ -196,7 +196,8 @@ class BatchedDot:
         otherwise get less than MIN_FILTER_WORK (or its largest TOD's work)."""
         if not work:
             return bool(tiled_gather is not False), 0
-        free = int(torch.cuda.mem_get_info()[0])
+        if free is None:
+            free = int(torch.cuda.mem_get_info()[0])
         left = free - MEM_RESERVE
         total, top = sum(work), max(work)
         if tiled_gather is None:

@@ -41,3 +41,26 @@ def test_groups_respect_cap_and_balance():
     # the largest TOD alone fills a group
     g = filter_groups([1, 50, 1], 50)
     _check([1, 50, 1], 50, g)
+
+
+def test_memory_plan_decisions():
+    """The tiled gather's index copy is kept while the filter still gets at
+    least MIN_FILTER_WORK (or its largest TOD's work), dropped before that,
+    and too little memory for even the largest TOD's filter work raises."""
+    from minkasi_amd.mapmaking.batch import MEM_RESERVE, MIN_FILTER_WORK, BatchedDot
+    GB = 1 << 30
+    work = [3 * GB, 2 * GB, 4 * GB] * 10          # 90 GiB of filter work in all, 4 GiB for the largest TOD
+    nsamp = 8 * 10 ** 9                            # 32 GB of tiled-gather indices
+    plan = BatchedDot._memory_plan
+    tiled, cap = plan(work, nsamp, None, None, free=MEM_RESERVE + 4 * nsamp + 20 * GB)
+    assert tiled and cap == 20 * GB
+    tiled, cap = plan(work, nsamp, None, None, free=MEM_RESERVE + 4 * nsamp + MIN_FILTER_WORK - 1)
+    assert not tiled and cap == 4 * nsamp + MIN_FILTER_WORK - 1
+    tiled, cap = plan(work, nsamp, True, 5 * GB, free=MEM_RESERVE + 4 * nsamp + 20 * GB)
+    assert tiled and cap == 5 * GB                 # the caller's cap wins when smaller
+    tiled, cap = plan(work, nsamp, False, None, free=MEM_RESERVE + 200 * GB)
+    assert not tiled and cap == 200 * GB
+    with pytest.raises(MemoryError):
+        plan(work, nsamp, None, None, free=MEM_RESERVE + 4 * GB - 1)
+    with pytest.raises(MemoryError):               # forced tiled gather leaves too little
+        plan(work, nsamp, True, None, free=MEM_RESERVE + 4 * nsamp + GB)
