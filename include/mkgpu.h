@@ -93,6 +93,14 @@ int mk_map2tod_f64(double *dat, const double *map, int64_t nsamp,
  * mk_map_to_tiles_* copies a row-major map into the tiled array (padding
  * zero).  mk_map2tod_* on (tiled map, remapped indices) equals mk_map2tod_*
  * on (map, indices) bit for bit. */
+/* Pixel-index validation (no reference counterpart: the reference's
+ * map2tod_omp / tod2map_* index the map unchecked, minkasi/lib/minkasi.c:
+ * 50-144).  Adds to *dev_nbad (device int64, caller-zeroed) the number of
+ * the nsamp indices outside [0, npix).  The Python layer runs it once per
+ * saved pixelization before any map2tod / atomic tod2map reads it. */
+int mk_pix_count_bad(const int32_t *ipix, int64_t nsamp, int64_t npix,
+                     int64_t *dev_nbad, void *stream);
+
 int64_t mk_map_tiled_size(int32_t nx, int32_t ny, int32_t tile_x,
                           int32_t tile_y);
 int mk_pix_tile_remap(const int32_t *ipix, int64_t nsamp, int32_t nx,
@@ -134,6 +142,16 @@ int mk_t2m_plan_exec_f32(const mk_t2m_plan *plan, float *map, const float *dat,
                          int accumulate, void *stream);
 int mk_t2m_plan_exec_f64(const mk_t2m_plan *plan, double *map,
                          const double *dat, int accumulate, void *stream);
+/* mk_t2m_plan_exec_f32 with the sums kept in FP64: FP32 samples, FP64
+ * segmented scans, LDS tiles and partial tiles (FP32 partials when the plan
+ * could not afford 8-byte ones), rounded once into the map, which is FP32
+ * (map_f64 = 0) or FP64 (map_f64 = 1).  Same plan, same fixed order, so
+ * bitwise repeatable.  The reference sums in FP64 (minkasi.c:97-112 on
+ * double maps); FP32 sums of a pixel's many cancelling samples were the
+ * largest error of the FP32 operator at the PCG solution. */
+int mk_t2m_plan_exec_f32_acc64(const mk_t2m_plan *plan, void *map,
+                               int map_f64, const float *dat, int accumulate,
+                               void *stream);
 /* info[0..7] = nsamp, nitems, ntiles_touched, tile_x, tile_y, plan bytes,
  * partial-buffer bytes, max samples per item. */
 int mk_t2m_plan_info(const mk_t2m_plan *plan, int64_t *info);
@@ -195,6 +213,24 @@ int mk_pcg_p(float *p, const float *z, int64_t n, const double *sc,
              void *stream);
 int mk_pcg_init(const float *r, const float *z, float *p, int64_t n,
                 double *sc, void *work, void *stream);
+
+/* Reliable updates for the true residual ||b - Ax||/||b|| (pcg.py:143-170
+ * keeps x and r in FP64 numpy maps; here they are FP32 device maps):
+ *  mk_pcg_xr_x64 / mk_pcg_xrz_jacobi_x64: as mk_pcg_xr / mk_pcg_xrz_jacobi,
+ *      with x accumulated in the FP64 map x64 (x receives its FP32 rounding),
+ *      so the iterate keeps the precision the FP32 map would lose.
+ *  mk_pcg_set_r: r = fl32(b - ax) from the FP64 right-hand side b and an FP64
+ *      product ax = A x64 (residual replacement), rr = r.r into the scalar
+ *      block.  After it the recursive residual starts again from the true
+ *      one, so its drift from the FP32 operator's rounding is reset. */
+int mk_pcg_xr_x64(double *x64, float *x, float *r, const float *p,
+                  const float *Ap, int64_t n, double *sc, void *work,
+                  void *stream);
+int mk_pcg_xrz_jacobi_x64(double *x64, float *x, float *r, float *z,
+                          const float *p, const float *Ap, const float *w,
+                          int64_t n, double *sc, void *work, void *stream);
+int mk_pcg_set_r(float *r, const double *b, const double *ax, int64_t n,
+                 double *sc, void *work, void *stream);
 
 
 /* ------------------------------------------------------------------ */

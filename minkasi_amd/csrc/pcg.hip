@@ -17,6 +17,7 @@ enum Mode { kDot = 0, kAlpha, kXR, kXRZJ, kBeta, kInit };
 
 struct Args {
   float* x; float* r; float* z; float* p;
+  double* x64;  // when set, x accumulates here in FP64 and x holds its FP32 rounding
   const float* Ap; const float* w;
   const float* a; const float* b;
   const double* a64; const double* b64;
@@ -51,7 +52,13 @@ __global__ void __launch_bounds__(kBlock) stage1(Args A, double* __restrict__ pa
       s0 += (double)A.p[i] * (double)A.Ap[i];
     } else if (M == kXR || M == kXRZJ) {
       float pi = A.p[i], api = A.Ap[i];
-      A.x[i] = (float)((double)A.x[i] + alpha * (double)pi);
+      if (A.x64) {
+        const double xi = A.x64[i] + alpha * (double)pi;
+        A.x64[i] = xi;
+        A.x[i] = (float)xi;
+      } else {
+        A.x[i] = (float)((double)A.x[i] + alpha * (double)pi);
+      }
       float ri = (float)((double)A.r[i] - alpha * (double)api);
       A.r[i] = ri;
       s1 += (double)ri * (double)ri;
@@ -306,6 +313,55 @@ extern "C" int mk_pcg_init(const float* r, const float* z, float* p, int64_t n, 
   Args A{};
   A.r = const_cast<float*>(r); A.z = const_cast<float*>(z); A.p = p; A.n = n;
   return run<kInit>(A, sc, work, stream);
+}
+
+namespace {
+// r = fl32(b - Ax) from FP64 b and Ax; rr = r.r (FP64) into the scalar block.
+__global__ void __launch_bounds__(kBlock) set_r_stage1(float* __restrict__ r, const double* __restrict__ b,
+                                                       const double* __restrict__ ax, int64_t n,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  double s1 = 0.0;
+  const int64_t stride = (int64_t)kGroups * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const float ri = (float)(b[i] - ax[i]);
+    r[i] = ri;
+    s1 += (double)ri * (double)ri;
+  }
+  s1 = block_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = 0.0;
+    part[kGroups + blockIdx.x] = s1;
+  }
+}
+}  // namespace
+
+extern "C" int mk_pcg_set_r(float* r, const double* b, const double* ax, int64_t n, double* sc, void* work,
+                            void* stream_v) {
+  MK_REQUIRE(n >= 0 && sc && work, MK_ERR_ARG, "bad arguments");
+  MK_REQUIRE(n == 0 || (r && b && ax), MK_ERR_ARG, "null input");
+  hipStream_t s = mk::as_stream(stream_v);
+  set_r_stage1<<<kGroups, kBlock, 0, s>>>(r, b, ax, n, (double*)work);
+  int st = mk::check_launch("set_r stage1");
+  if (st) return st;
+  stage2<kXR><<<1, kBlock, 0, s>>>((double*)work, sc);
+  return mk::check_launch("set_r stage2");
+}
+
+extern "C" int mk_pcg_xr_x64(double* x64, float* x, float* r, const float* p, const float* Ap, int64_t n,
+                             double* sc, void* work, void* stream) {
+  MK_REQUIRE(n == 0 || (x64 && x && r && p && Ap), MK_ERR_ARG, "null input");
+  Args A{};
+  A.x64 = x64; A.x = x; A.r = r; A.p = const_cast<float*>(p); A.Ap = Ap; A.sc = sc; A.n = n;
+  return run<kXR>(A, sc, work, stream);
+}
+
+extern "C" int mk_pcg_xrz_jacobi_x64(double* x64, float* x, float* r, float* z, const float* p, const float* Ap,
+                                     const float* w, int64_t n, double* sc, void* work, void* stream) {
+  MK_REQUIRE(n == 0 || (x64 && x && r && z && p && Ap && w), MK_ERR_ARG, "null input");
+  Args A{};
+  A.x64 = x64; A.x = x; A.r = r; A.z = z; A.p = const_cast<float*>(p); A.Ap = Ap; A.w = w; A.sc = sc; A.n = n;
+  return run<kXRZJ>(A, sc, work, stream);
 }
 
 namespace {

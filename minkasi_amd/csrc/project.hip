@@ -145,6 +145,17 @@ __global__ void __launch_bounds__(256) pix_tile_kernel(const int32_t* __restrict
   }
 }
 
+// Counts indices outside [0, npix) (one wave ballot and at most one atomic
+// per wave, mk::count_bad).  Reads 4 B/sample and writes nothing else.
+__global__ void __launch_bounds__(256) pix_count_bad_kernel(const int32_t* __restrict__ ipix, int64_t n,
+                                                            int64_t npix, int64_t* nbad) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t p = ipix[i];
+    mk::count_bad(p < 0 || (int64_t)p >= npix, nbad);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) map_tiles_kernel(const T* __restrict__ map, int32_t nx, int32_t ny, int32_t tx,
                                                         int32_t ty, T* __restrict__ tiled, int64_t ntiled) {
@@ -184,6 +195,17 @@ int tod2map_atomic_impl(T* map, const T* dat, int64_t nsamp, const int32_t* ipix
 }
 
 }  // namespace
+
+extern "C" int mk_pix_count_bad(const int32_t* ipix, int64_t nsamp, int64_t npix, int64_t* dev_nbad,
+                                void* stream) {
+  MK_REQUIRE(nsamp >= 0 && npix >= 0, MK_ERR_ARG, "nsamp or npix < 0");
+  MK_REQUIRE(dev_nbad, MK_ERR_ARG, "null counter");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(ipix, MK_ERR_ARG, "null pointer");
+  pix_count_bad_kernel<<<mk::grid_for(nsamp, 256, 256 * 16), 256, 0, mk::as_stream(stream)>>>(ipix, nsamp, npix,
+                                                                                              dev_nbad);
+  return mk::check_launch("pix_count_bad_kernel");
+}
 
 extern "C" int mk_map2tod_f32(float* dat, const float* map, int64_t nsamp, const int32_t* ipix, int do_add,
                               void* stream) {
@@ -359,11 +381,13 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
 // group, in group order: the result does not depend on timing).
 constexpr int kT2MUnroll = 16;
 
-template <typename T, typename P>
+// TD: sample type; T: accumulation type (FP64 for FP32 samples keeps the
+// per-pixel sums of many cancelling samples at FP64 accuracy); P: partials.
+template <typename TD, typename T, typename P>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
                                                         const uint32_t* __restrict__ entry,
                                                         const uint16_t* __restrict__ lpix,
-                                                        const T* __restrict__ dat, int tile_px,
+                                                        const TD* __restrict__ dat, int tile_px,
                                                         P* __restrict__ partial) {
   extern __shared__ unsigned char smem_raw[];
   T* tiles = reinterpret_cast<T*>(smem_raw);  // [4][tile_px]
@@ -372,7 +396,7 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
   for (int p = threadIdx.x; p < 4 * tile_px; p += 256) tiles[p] = T(0);
   __syncthreads();
   T* mine = tiles + wave * tile_px;
-  const T* base = dat + it.seg_base;
+  const TD* base = dat + it.seg_base;
   const int64_t end = it.begin + it.count;
   const int ngroups = (it.count + 63) >> 6;
   const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
@@ -393,7 +417,7 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
     }
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
-      const T x = base[e[j]];
+      const T x = (T)base[e[j]];
       v[j] = valid[j] ? x : T(0);
       lp[j] = valid[j] ? lp[j] : -1 - lane;  // distinct sentinels never match
     }
@@ -431,10 +455,12 @@ struct alignas(16) Vec4<float> { float v[4]; };
 template <>
 struct alignas(16) Vec4<double> { double v[4]; };
 
-template <typename T, typename P>
+// T: accumulation type, TM: map type (an FP32 map summed in FP64 is rounded
+// once, when stored).
+template <typename T, typename P, typename TM>
 __global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restrict__ tile_items,
                                                         const P* __restrict__ partial, int tile_x, int tile_y,
-                                                        int ntile_y, int nx, int ny, T* __restrict__ map,
+                                                        int ntile_y, int nx, int ny, TM* __restrict__ map,
                                                         int accumulate, int blocks_per_tile) {
   const int tile = blockIdx.x / blocks_per_tile;
   const int tile_px = tile_x * tile_y;
@@ -463,8 +489,8 @@ __global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restric
   for (int k = 0; k < 4; k++) {
     const int y = y0 + k;
     if (y >= ny) break;
-    T* m = map + (int64_t)x * ny + y;
-    *m = accumulate ? *m + acc[k] : acc[k];
+    TM* m = map + (int64_t)x * ny + y;
+    *m = (TM)(accumulate ? (T)*m + acc[k] : acc[k]);
   }
 }
 
@@ -617,8 +643,9 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
 }
 
 namespace {
-template <typename T>
-int t2m_exec(const mk_t2m_plan* pl, T* map, const T* dat, int accumulate, void* stream_v) {
+// TD: sample type, T: accumulation type, TM: map type.
+template <typename TD, typename T, typename TM>
+int t2m_exec(const mk_t2m_plan* pl, TM* map, const TD* dat, int accumulate, void* stream_v) {
   MK_REQUIRE(pl && map, MK_ERR_ARG, "null plan/map");
   MK_REQUIRE(pl->nsamp == 0 || dat, MK_ERR_ARG, "null dat");
   hipStream_t stream = mk::as_stream(stream_v);
@@ -626,39 +653,47 @@ int t2m_exec(const mk_t2m_plan* pl, T* map, const T* dat, int accumulate, void* 
   const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
   const size_t lds = 4 * (size_t)tile_px * sizeof(T);
   MK_REQUIRE(lds <= 65536, MK_ERR_ARG, "tile too large for LDS with this precision");
+  // FP64 partial tiles whenever the plan holds them (8-byte partial
+  // elements); an FP64 accumulation with 4-byte partials rounds each item's
+  // sum once
+  const bool p64 = sizeof(T) == 8 && pl->partial_elem_bytes == 8;
+  MK_REQUIRE(p64 || sizeof(TD) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
   if (pl->nitems > 0) {
-    if (sizeof(T) == 8 && pl->partial_elem_bytes == 8) {
-      t2m_items_kernel<T, double><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix, dat,
-                                                                              tile_px, (double*)pl->partial);
-    } else {
-      MK_REQUIRE(sizeof(T) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
-      t2m_items_kernel<T, float><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix, dat,
-                                                                             tile_px, (float*)pl->partial);
-    }
+    if (p64)
+      t2m_items_kernel<TD, T, double><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix,
+                                                                                  dat, tile_px, (double*)pl->partial);
+    else
+      t2m_items_kernel<TD, T, float><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix,
+                                                                                 dat, tile_px, (float*)pl->partial);
     int st = mk::check_launch("t2m_items_kernel");
     if (st) return st;
   }
   const int bpt = (int)mk::cdiv(tile_px / 4, 256);
   const unsigned nblk = (unsigned)(ntiles * bpt);
-  if (sizeof(T) == 8 && pl->partial_elem_bytes == 8)
-    t2m_tiles_kernel<T, double><<<nblk, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial, pl->tile_x,
-                                                          pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map, accumulate,
-                                                          bpt);
+  if (p64)
+    t2m_tiles_kernel<T, double, TM><<<nblk, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial, pl->tile_x,
+                                                              pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map,
+                                                              accumulate, bpt);
   else
-    t2m_tiles_kernel<T, float><<<nblk, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial, pl->tile_x,
-                                                         pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map, accumulate,
-                                                         bpt);
+    t2m_tiles_kernel<T, float, TM><<<nblk, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial, pl->tile_x,
+                                                             pl->tile_y, pl->ntile_y, pl->nx, pl->ny, map,
+                                                             accumulate, bpt);
   return mk::check_launch("t2m_tiles_kernel");
 }
 }  // namespace
 
 extern "C" int mk_t2m_plan_exec_f32(const mk_t2m_plan* plan, float* map, const float* dat, int accumulate,
                                     void* stream) {
-  return t2m_exec<float>(plan, map, dat, accumulate, stream);
+  return t2m_exec<float, float, float>(plan, map, dat, accumulate, stream);
 }
 extern "C" int mk_t2m_plan_exec_f64(const mk_t2m_plan* plan, double* map, const double* dat, int accumulate,
                                     void* stream) {
-  return t2m_exec<double>(plan, map, dat, accumulate, stream);
+  return t2m_exec<double, double, double>(plan, map, dat, accumulate, stream);
+}
+extern "C" int mk_t2m_plan_exec_f32_acc64(const mk_t2m_plan* plan, void* map, int map_f64, const float* dat,
+                                          int accumulate, void* stream) {
+  if (map_f64) return t2m_exec<float, double, double>(plan, (double*)map, dat, accumulate, stream);
+  return t2m_exec<float, double, float>(plan, (float*)map, dat, accumulate, stream);
 }
 extern "C" int mk_t2m_plan_info(const mk_t2m_plan* pl, int64_t* info) {
   MK_REQUIRE(pl && info, MK_ERR_ARG, "null pointer");

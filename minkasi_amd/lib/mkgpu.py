@@ -65,6 +65,7 @@ _SIGS = {
     "mk_t2m_plan_create": ([P, I64, I32, I32, I32, I32, I32, ctypes.POINTER(P), P], I),
     "mk_t2m_plan_exec_f32": ([P, P, P, I, P], I),
     "mk_t2m_plan_exec_f64": ([P, P, P, I, P], I),
+    "mk_t2m_plan_exec_f32_acc64": ([P, P, I, P, I, P], I),
     "mk_t2m_plan_info": ([P, P], I),
     "mk_t2m_plan_destroy": ([P], I),
     "mk_dot_f32": ([P, P, I64, P, P, P], I),
@@ -78,6 +79,9 @@ _SIGS = {
     "mk_pcg_p": ([P, P, I64, P, P], I),
     "mk_project_out_f32": ([P, P, I64, D, P, P, P], I),
     "mk_pcg_init": ([P, P, P, I64, P, P, P], I),
+    "mk_pcg_xr_x64": ([P, P, P, P, P, I64, P, P, P], I),
+    "mk_pcg_xrz_jacobi_x64": ([P, P, P, P, P, P, P, I64, P, P, P], I),
+    "mk_pcg_set_r": ([P, P, P, I64, P, P, P], I),
     "mk_dct1_workspace_bytes": ([I64, I64, I], I64),
     "mk_dct1_f64": ([P, P, I64, I64, P, I64, P], I),
     "mk_dct1_f32": ([P, P, I64, I64, P, I64, P], I),
@@ -119,6 +123,7 @@ _SIGS = {
     "mk_addmul_f64": ([P, P, P, I64, P], I),
     "mk_filt_plan_create_p1": ([I, P, I, P, P], I),
     "mk_filt_hot_p1": ([P, I], I),
+    "mk_pix_count_bad": ([P, I64, I64, P, P], I),
     "mk_map_tiled_size": ([I32, I32, I32, I32], I64),
     "mk_pix_tile_remap": ([P, I64, I32, I32, I32, I32, P, P, P], I),
     "mk_map_to_tiles_f32": ([P, I32, I32, I32, I32, P, P], I),

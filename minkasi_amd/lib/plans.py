@@ -2,6 +2,7 @@
 rotation).  Each handle keeps the tensors its device descriptors point at
 alive and destroys the plan when collected."""
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -29,6 +30,11 @@ class _Handle:
         self.close()
 
 
+# FP32 samples are binned with FP64 sums unless MK_T2M_ACC32 is set (the
+# FP32-sum kernel is kept for comparison).
+ACC64_DEFAULT = not os.environ.get("MK_T2M_ACC32")
+
+
 class T2MPlan(_Handle):
     """Deterministic tod2map plan over a (batch) ipix array."""
 
@@ -52,16 +58,34 @@ class T2MPlan(_Handle):
         keys = ["nsamp", "nitems", "ntiles_touched", "tile_x", "tile_y", "plan_bytes", "partial_bytes", "max_item"]
         return dict(zip(keys, (int(v) for v in out)))
 
-    def exec(self, map_t, dat, accumulate):
+    def exec(self, map_t, dat, accumulate, acc64=None):
+        """map_t (+)= tod2map(dat) in the plan's fixed order.
+
+        FP32 samples are summed in FP64 (acc64, the default when
+        MK_T2M_ACC32 is unset) into an FP32 or FP64 map; acc64=False keeps
+        the FP32 sums of mk_t2m_plan_exec_f32.  FP64 samples need an FP64
+        map and are summed in FP64."""
         if map_t.numel() != self.nx * self.ny:
             raise ValueError("map size %d does not match plan %dx%d" % (map_t.numel(), self.nx, self.ny))
         if dat.numel() != self.nsamp:
             raise ValueError("TOD has %d samples, plan expects %d" % (dat.numel(), self.nsamp))
-        if map_t.dtype != dat.dtype:
-            raise TypeError("map and TOD dtypes differ (%s vs %s)" % (map_t.dtype, dat.dtype))
-        name = "mk_t2m_plan_exec_f32" if map_t.dtype == torch.float32 else "mk_t2m_plan_exec_f64"
         dptr = dev.ptr(dat, name="dat") if self.nsamp else dev.ptr(map_t)
-        mkgpu.check(name, self.h, dev.ptr(map_t, name="map"), dptr, int(bool(accumulate)), dev.stream_ptr())
+        mptr = dev.ptr(map_t, name="map")
+        acc = int(bool(accumulate))
+        if dat.dtype == torch.float64:
+            if map_t.dtype != torch.float64:
+                raise TypeError("FP64 samples need an FP64 map (got %s)" % map_t.dtype)
+            mkgpu.check("mk_t2m_plan_exec_f64", self.h, mptr, dptr, acc, dev.stream_ptr())
+            return
+        if dat.dtype != torch.float32 or map_t.dtype not in (torch.float32, torch.float64):
+            raise TypeError("unsupported tod2map dtypes (map %s, TOD %s)" % (map_t.dtype, dat.dtype))
+        if acc64 is None:
+            acc64 = ACC64_DEFAULT
+        if map_t.dtype == torch.float64 or acc64:
+            mkgpu.check("mk_t2m_plan_exec_f32_acc64", self.h, mptr, int(map_t.dtype == torch.float64), dptr, acc,
+                        dev.stream_ptr())
+        else:
+            mkgpu.check("mk_t2m_plan_exec_f32", self.h, mptr, dptr, acc, dev.stream_ptr())
 
 
 class FiltPlan(_Handle):

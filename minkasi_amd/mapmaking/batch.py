@@ -124,9 +124,17 @@ class BatchedDot:
             ip = skymap.get_pix(tod)
             self.ipix[o:o + ndet * n].copy_(ip.reshape(-1))
             tod.info[self.tag] = self.ipix[o:o + ndet * n].view(ndet, n)
-        self.ipix_ids = [id(tod.info[self.tag]) for tod in self.tods]
-        self.buf_a = torch.empty(max(off, 4), dtype=torch.float32, device=d)
-        self.buf_b = torch.empty(max(off, 4), dtype=torch.float32, device=d)
+        # what the operator was built from: the TODs' pixelizations (views into
+        # the arena) and noise models; matches() rebuilds when either changes
+        self.ipix_views = [tod.info[self.tag] for tod in self.tods]
+        self.noises = [tod.noise for tod in self.tods]
+        # the two FP32 work arenas are halves of one block, which the FP64
+        # operator of the true-residual check (AccurateDot) reuses as one FP64
+        # arena; each half starts 256-byte aligned
+        half = -(-max(off, 4) // 64) * 64
+        self._bufs = torch.empty(2 * half, dtype=torch.float32, device=d)
+        self.buf_a = self._bufs[:half]
+        self.buf_b = self._bufs[half:]
         # the per-TOD index blocks the arena replaced go back to the device,
         # where the library's own allocations can use them
         torch.cuda.empty_cache()
@@ -186,6 +194,11 @@ class BatchedDot:
             tod.noise.filt32.release_work()
         self.graph = None
         self._gx = self._gout = None
+        # tod2map sums of A.x: FP32 (3.7 ms at 1.9e9 samples) unless set;
+        # into an FP64 map (f64_view) and for the right-hand side they are
+        # FP64 (6.1 ms); the PCG iteration needs no more than FP32 sums
+        # because its residual is replaced from the FP64-binned check
+        self.t2m_acc64 = False
 
     @staticmethod
     def _memory_plan(work, nsamp, tiled_gather, filter_work_cap, free=None):
@@ -260,12 +273,13 @@ class BatchedDot:
         """a = x.flat[ipix] over the arena, gathered from the tiled copy (or
         from the row-major map when memory is short)."""
         if not self.tiled:
-            map2tod(a, x.reshape(-1), self.ipix[:self.nsamp], do_add=False)
+            # the arena's indices were checked against nx * ny at construction
+            map2tod(a, x.reshape(-1), self.ipix[:self.nsamp], do_add=False, check=False)
             return
         tx, ty = self.gather_tile
         mkgpu.check("mk_map_to_tiles_f32", dev.ptr(x, torch.float32, "map"), self.nx, self.ny, tx, ty,
                     self.map_t.data_ptr(), dev.stream_ptr())
-        map2tod(a, self.map_t, self.ipix_g[:self.nsamp], do_add=False)
+        map2tod(a, self.map_t, self.ipix_g[:self.nsamp], do_add=False, check=False)
 
     @staticmethod
     def _time(fn, reps=3):
@@ -279,9 +293,18 @@ class BatchedDot:
         return e0.elapsed_time(e1) / reps
 
     def matches(self, todvec, mapset):
+        """Same TODs, map size, pixelizations and noise models.  A TOD whose
+        noise was re-estimated (Tod.set_noise, as the reference's multi-pass
+        examples do between solves) or whose pixelization was replaced makes
+        the cached operator stale."""
         m = mapset.maps[0]
-        return (len(todvec.tods) == len(self.tods) and all(a is b for a, b in zip(todvec.tods, self.tods))
-                and m.nx == self.nx and m.ny == self.ny)
+        if not (len(todvec.tods) == len(self.tods) and all(a is b for a, b in zip(todvec.tods, self.tods))
+                and m.nx == self.nx and m.ny == self.ny and m.tag == self.tag):
+            return False
+        for tod, ip, nz in zip(self.tods, self.ipix_views, self.noises):
+            if tod.noise_delayed or tod.noise is not nz or tod.info.get(self.tag) is not ip:
+                return False
+        return True
 
     # ------------------------------------------------------------------
     def stages(self, x, out, accumulate):
@@ -291,7 +314,7 @@ class BatchedDot:
         self._rot_v(a, b)
         self._filter(b)
         self.rot_vT.exec(b, a)
-        self.t2m.exec(out.reshape(-1), a, accumulate)
+        self.t2m.exec(out.reshape(-1), a, accumulate, acc64=self.t2m_acc64)
 
     def apply(self, x, out, accumulate=True):
         """out (+)= A x for device maps x, out (nx, ny) FP32."""
@@ -328,7 +351,7 @@ class BatchedDot:
         self._rot_v(a, b)
         self._filter(b)
         self.rot_vT.exec(b, a)
-        self.t2m.exec(out.reshape(-1), a, accumulate)
+        self.t2m.exec(out.reshape(-1), a, accumulate, acc64=True)
         return out
 
     def hits(self, out, accumulate=True):
@@ -337,6 +360,31 @@ class BatchedDot:
         self.buf_a[:n].fill_(1.0)
         self.t2m.exec(out.reshape(-1), self.buf_a[:n], accumulate)
         return out
+
+    def null_mask(self):
+        """Hit pixels (bool, nx x ny): the support of A's null vector, the
+        constant on hit pixels (computed once with the tod2map plan)."""
+        m = getattr(self, "_null", None)
+        if m is None:
+            h = torch.zeros((self.nx, self.ny), dtype=torch.float32, device=self.buf_a.device)
+            self.hits(h, accumulate=False)
+            m = self._null = h > 0
+        return m
+
+    def accurate(self):
+        """The all-FP64 operator over these TODs (AccurateDot, built once)."""
+        a = getattr(self, "_accurate", None)
+        if a is None:
+            a = self._accurate = AccurateDot(self)
+        return a
+
+    def f64_view(self):
+        """This operator with FP64 binning into FP64 maps (apply(x64, out64),
+        rhs(out64)): the FP32 stages see fl32(x64), tod2map sums in FP64."""
+        v = getattr(self, "_f64", None)
+        if v is None:
+            v = self._f64 = _Binned64(self)
+        return v
 
     def stage_times(self, x, out, reps=3):
         """Per-stage device time (ms) with events on the launch stream."""
@@ -347,7 +395,7 @@ class BatchedDot:
             ("rot_v", lambda: self._rot_v(a, b)),
             ("filter", lambda: self._filter(b)),
             ("rot_vT", lambda: self.rot_vT.exec(b, a)),
-            ("tod2map", lambda: self.t2m.exec(out.reshape(-1), a, False)),
+            ("tod2map", lambda: self.t2m.exec(out.reshape(-1), a, False, acc64=self.t2m_acc64)),
         ]
         res = {}
         for name, fn in steps:
@@ -360,6 +408,27 @@ class BatchedDot:
             e1.synchronize()
             res[name] = e0.elapsed_time(e1) / reps
         return res
+
+
+class _Binned64:
+    """BatchedDot.f64_view: the FP32 operator evaluated from an FP64 map
+    into an FP64 map, its tod2map summing in FP64."""
+
+    def __init__(self, base):
+        self.base = base
+        self.nx, self.ny = base.nx, base.ny
+        self.x32 = torch.empty((base.nx, base.ny), dtype=torch.float32, device=base.buf_a.device)
+
+    def matches(self, base):
+        return base is self.base
+
+    def apply(self, x64, out64):
+        self.x32.copy_(x64)
+        self.base.stages(self.x32, out64, False)
+        return out64
+
+    def rhs(self, out64):
+        return self.base.rhs(out64, accumulate=False)
 
 
 class BatchedCutsDot:
@@ -420,6 +489,16 @@ class BatchedCutsDot:
             if id(c.imap) != self.cut_ids[j] and not torch.equal(c.imap, self._imap_of(j)):
                 return False
         return True
+
+    def null_mask(self):
+        """[hit pixels | no cut samples]: A's null vector is the constant on
+        hit pixels and zero on the cut samples."""
+        m = getattr(self, "_null", None)
+        if m is None:
+            m = torch.zeros(self.n, dtype=torch.bool, device=self.base.buf_a.device)
+            m[:self.npix] = self.base.null_mask().reshape(-1)
+            self._null = m
+        return m
 
     def _imap_of(self, j):
         o = sum(self.counts[:j])
@@ -491,7 +570,7 @@ class BatchedCutsDot:
         B._rot_v(a, b)
         B._filter(b)
         B.rot_vT.exec(b, a)
-        B.t2m.exec(out[:self.npix], a, accumulate)
+        B.t2m.exec(out[:self.npix], a, accumulate, acc64=B.t2m_acc64)
         self._cuts_out(a, out[self.npix:], accumulate)
         return out
 
@@ -502,3 +581,103 @@ class BatchedCutsDot:
         B.rhs(out[:self.npix].view(B.nx, B.ny), accumulate)  # leaves N^-1 d in buf_a
         self._cuts_out(B.buf_a[:n], out[self.npix:], accumulate)
         return out
+
+
+class AccurateDot:
+    """FP64 A.x and right-hand side over the TODs of a BatchedDot: the
+    operator the true residual ||b - A x||/||b|| is measured with, and the
+    one the device PCG replaces its recursive residual from.
+
+    The reference evaluates all of A in FP64 (minkasi/tods/core.py:649-691
+    on numpy maps, noise.py:680-714).  The batched FP32 operator differs from
+    it by a few 1e-7 relative at the solution, as much as the residual a
+    solve to 1e-6 is asked to reach, so the check runs every stage in FP64:
+
+      map2tod   FP64 gather over the arena           (mk_map2tod_f64)
+      v.d       FP64 GEMM per TOD                    (mk_gemm_nn_f64)
+      C W C     FP64 filter, batched plans borrowing the noise models' FP64
+                spectra, TODs in groups sharing one work buffer
+      vT.(.)    FP64 GEMM per TOD
+      tod2map   FP64 through the batched deterministic plan
+
+    Every TOD's noise model must keep its FP64 filter (keep_fp64=True).  The
+    FP64 arena aliases the FP32 operator's two work arenas (the two operators
+    never run at the same time on the launch stream); a second FP64 arena is
+    this object's own (8 bytes per sample)."""
+
+    def __init__(self, base, work_cap=8 << 30):
+        missing = [k for k, tod in enumerate(base.tods) if getattr(tod.noise, "filt64", None) is None]
+        if missing:
+            raise ValueError("AccurateDot needs noise models built with keep_fp64=True (TODs %s lack the FP64 "
+                             "filter)" % missing[:8])
+        self.base = base
+        self.nx, self.ny = base.nx, base.ny
+        n = base.nsamp
+        d = base.buf_a.device
+        self.a64 = base._bufs.view(torch.float64)
+        if self.a64.numel() < n:
+            raise RuntimeError("FP64 view of the work arenas is shorter than the arena")
+        self.b64 = torch.empty(max(self.a64.numel(), 4), dtype=torch.float64, device=d)
+        self.v64 = [tod.noise._v64 for tod in base.tods]
+        self.vT64 = [tod.noise._vT64 for tod in base.tods]
+        work = [int(tod.noise.filt64.work_bytes) for tod in base.tods]
+        for tod in base.tods:
+            tod.noise.filt64.release_work()
+        cap = max(max(work, default=0), min(int(work_cap), sum(work)))
+        self.filter_groups = filter_groups(work, cap)
+        self.filts = []
+        for grp in self.filter_groups:
+            segs = [(base.sizes[i][0], base.sizes[i][1], base.offs[i], base.sizes[i][1], base.offs[i],
+                     base.sizes[i][1]) for i in grp]
+            fp = FiltPlan(segs, 8)
+            for k, i in enumerate(grp):
+                fp.borrow(k, base.tods[i].noise.filt64, 0)
+            fp.set_halve(True)
+            self.filts.append(fp)
+        wmax = max((sum(work[i] for i in g) for g in self.filter_groups), default=0)
+        self.filter_work = torch.empty(max(wmax, 16), dtype=torch.uint8, device=d)
+        for fp in self.filts:
+            fp.set_work(self.filter_work)
+
+    def matches(self, base):
+        return base is self.base
+
+    def _rotate(self, mats, src, dst):
+        sp = dev.stream_ptr()
+        for m, o, (ndet, n) in zip(mats, self.base.offs, self.base.sizes):
+            mkgpu.check("mk_gemm_nn_f64", ndet, n, ndet, m.data_ptr(), ndet, src[o:o + ndet * n].data_ptr(), 0, n,
+                        dst[o:o + ndet * n].data_ptr(), n, sp)
+
+    def _noise(self, a, b):
+        """b = v.a ; b = C W C b ; a = vT.b (per TOD, FP64)."""
+        self._rotate(self.v64, a, b)
+        for fp in self.filts:
+            fp.exec(b, b)
+        self._rotate(self.vT64, b, a)
+
+    def apply(self, x64, out64):
+        """out64 = A x64 for FP64 device maps (nx, ny)."""
+        B = self.base
+        n = B.nsamp
+        a, b = self.a64[:n], self.b64[:n]
+        if x64.dtype != torch.float64 or out64.dtype != torch.float64:
+            raise TypeError("AccurateDot works on FP64 maps")
+        mkgpu.check("mk_map2tod_f64", a.data_ptr(), dev.ptr(x64, torch.float64, "map"), n, B.ipix.data_ptr(), 0,
+                    dev.stream_ptr())
+        self._noise(a, b)
+        B.t2m.exec(out64.reshape(-1), a, False)
+        return out64
+
+    def rhs(self, out64):
+        """out64 = sum over TODs of P^T N^-1 d in FP64 (TodVec.make_rhs,
+        minkasi/tods/core.py:1028-1047)."""
+        B = self.base
+        n = B.nsamp
+        a, b = self.a64[:n], self.b64[:n]
+        for tod, o, (ndet, nn) in zip(B.tods, B.offs, B.sizes):
+            dd = tod.get_data()
+            dd = dd if isinstance(dd, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(dd))
+            a[o:o + ndet * nn].copy_(dd.reshape(-1).to(a.device, torch.float64))
+        self._noise(a, b)
+        B.t2m.exec(out64.reshape(-1), a, False)
+        return out64

@@ -15,6 +15,10 @@ def tod2map_omp(map, dat, ipix, atomic=False):
     if ipix.dtype != torch.int32:
         print("Warning - ipix is not int32 in tod2map_omp.  this is likely to produce garbage results.")
     if atomic:
+        from .map2tod import check_pix
+        if dat.numel() != ipix.numel():
+            raise ValueError("dat has %d samples, ipix %d" % (dat.numel(), ipix.numel()))
+        check_pix(ipix, map.numel())
         name = {torch.float32: "mk_tod2map_atomic_f32", torch.float64: "mk_tod2map_atomic_f64"}[dat.dtype]
         mkgpu.check(name, dev.ptr(map, name="map"), dev.ptr(dat, name="dat"), dat.numel(),
                     dev.ptr(ipix, torch.int32, "ipix"), dev.stream_ptr())

@@ -256,6 +256,10 @@ class SkyMap:
                 if not isinstance(ipix, torch.Tensor):
                     ipix = dev.to_device(np.asarray(ipix, dtype=np.int32))
                     tod.info[self.tag] = ipix
+                # a saved pixelization may belong to another map (the default
+                # tag is shared) or be user-supplied: checked once per tensor
+                from ..mapmaking.map2tod import check_pix
+                check_pix(ipix, self.nx * self.ny)
                 return ipix
         if "dx" not in tod.info and "ra_b" in tod.info:
             ipix = self.pix_from_raster(tod.info["ra_b"], tod.info["dec_b"], tod.info["dra"], tod.info["ddec"])

@@ -339,6 +339,11 @@ class TodVec:
             return None
         b = self._batch
         if b is None or not b.matches(self, mapset):
+            # a stale operator (other noise models, pixelization or map size)
+            # is released before the new one takes its device memory
+            self._batch = b = None
+            self._batch_cuts = None
+            torch.cuda.empty_cache()
             b = BatchedDot(self, mapset.maps[0], **kw)
             self._batch = b
         return b

@@ -15,9 +15,11 @@ class Problem:
     pass
 
 
-def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False, truncate=False):
+def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False, truncate=False, keep_fp64=False):
     """truncate: cut every TOD so that n - 1 is {2,3,5,7,11}-smooth, as the
-    reference's examples do before mapmaking (tods/processing.py:648-681)."""
+    reference's examples do before mapmaking (tods/processing.py:648-681).
+    keep_fp64: the noise models also keep their FP64 filters, for the FP64
+    operator that checks the solution's true residual."""
     from .maps.mapset import Mapset
     from .maps.skymap import SkyMap
     from .mapmaking.noise import NoiseSmoothedSVD
@@ -55,7 +57,7 @@ def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False,
     P.t_pixelize = time.time() - t0
     t0 = time.time()
     for k, tod in enumerate(todvec.tods):
-        tod.set_noise(NoiseSmoothedSVD, fwhm=c["fwhm"])
+        tod.set_noise(NoiseSmoothedSVD, fwhm=c["fwhm"], keep_fp64=True if keep_fp64 else None)
         if verbose and k % 16 == 0:
             print("noise model", k, "of", len(todvec.tods), flush=True)
     torch.cuda.synchronize()
@@ -85,17 +87,23 @@ def build_problem(config_id, ntod=None, seed_base=0, shapes=None, verbose=False,
     return P
 
 
-def solve_to_tol(P, precon=None, tol=1e-6, maxiter=600, graph=True):
-    """Device PCG from x0 = 0 until the recursive ||r||/||b|| < tol.
+def solve_to_tol(P, precon=None, tol=1e-6, maxiter=600, graph=True, check=None, b64=None, verify=None):
+    """Device PCG from x0 = 0 until ||r||/||b|| < tol.
 
     precon: None for the problem's hits-count Jacobi, else an object with
-    apply_to(r, z) (e.g. the learned preconditioner).  Returns a dict with
-    iterations, wall seconds (including init), final recursive and true
-    relative residuals, and the solution tensor."""
+    apply_to(r, z) (e.g. the learned preconditioner).  check/b64: a check
+    operator and the FP64 right-hand side (pcg.make_check): the iterate
+    accumulates in FP64, the residual is replaced once, and the solve stops
+    only when the true residual from the check operator is below tol (the
+    check is inside the timed region).  verify: another operator (e.g. the
+    all-FP64 AccurateDot) whose true residual of the final iterate is
+    reported as true_rel_res_fp64, outside the timed region.  Returns a dict
+    with iterations, wall seconds (including init), final recursive and true
+    relative residuals, and the solution tensor (FP64 when checked)."""
     import time
     from .mapmaking.pcg import DevicePCG
     op, b = P.op, P.rhs.maps[0].map
-    solver = DevicePCG(op, b, P.precon if precon is None else precon)
+    solver = DevicePCG(op, b, P.precon if precon is None else precon, check=check, b64=b64)
     x = torch.zeros_like(b)
     if graph:
         solver.init(x)
@@ -104,18 +112,30 @@ def solve_to_tol(P, precon=None, tol=1e-6, maxiter=600, graph=True):
     torch.cuda.synchronize()
     t0 = time.time()
     solver.init(x)
-    it, rel, hist = 0, 1.0, []
+    it, rel, hist, done = 0, 1.0, [], False
     while it < maxiter:
         solver.step()
         it += 1
         rel = solver.rel_res()
         hist.append(rel)
-        if rel < tol:
+        if solver.converged(tol, rel):
+            done = True
             break
     torch.cuda.synchronize()
     dt = time.time() - t0
-    return dict(iters=it, seconds=dt, rel_res=rel, reached=bool(rel < tol), true_rel_res=solver.true_rel_res(),
-                x=x, history=hist)
+    out = dict(iters=it, seconds=dt, rel_res=rel, reached=done)
+    if check is not None:
+        out.update(true_rel_res=solver.last_true if solver.last_true is not None else solver.true_rel_res64(),
+                   replacements=solver.replacements, checks=solver.checks, x=solver.x64)
+    else:
+        out.update(true_rel_res=solver.true_rel_res(), x=x)
+    if verify is not None:
+        if solver.x64 is None:
+            solver.x64 = x.to(torch.float64)
+            solver.b64 = b64
+        out["true_rel_res_fp64"] = solver.true_rel_res64(verify)
+    out["history"] = hist
+    return out
 
 
 def map_distance(P, x, y):

@@ -258,9 +258,18 @@ def test_run_pcg_matches_oracle():
     print("pcg 20 it: map rel err", e, "zr rel err last", abs(zr[-1] - zrr[-1]) / abs(zrr[-1]))
     assert e < 1e-3
     assert np.allclose(zr[:5], zrr[:5], rtol=1e-3)
-    # generic (Mapset-level) path gives the same iterates
-    x2 = mk.run_pcg(rhs, x0, todvec, pre, maxiter=20, verbose=False, mask=None, true_residual_every=0) \
-        if False else None
+    # the generic (Mapset-level) loop, forced by a mask of ones, runs the same
+    # update order with host scalars and gives the same iterates
+    ones = ms.copy()
+    ones.maps[0].map.fill_(1.0)
+    hist2 = []
+    x2 = mk.run_pcg(rhs, x0, todvec, pre, maxiter=20, verbose=False, mask=ones, history=hist2)
+    e2 = rel_err(x2.maps[0].map.cpu().numpy(), xg)
+    zr2 = [h_["zr"] for h_ in hist2]
+    print("generic vs device path: map rel err", e2)
+    assert len(zr2) == len(zr) == 20
+    assert e2 < 1e-4
+    assert np.allclose(zr2, zr, rtol=1e-4)
 
 
 def test_fp64_config0_path_matches_oracle():

@@ -123,7 +123,9 @@ def test_filter_batched_matches_dct_pair(env, dtype):
     for rows, n in segs:
         o = out[off:off + rows * n]
         r = ref[off:off + rows * n]
-        assert rel_err(o, r) < tol, (rows, n)
+        e, em = rel_err(o, r), max_rel_err(o, r)
+        print("filter", dtype.__name__, rows, n, "norm %.2e max %.2e" % (e, em))
+        assert e < tol and em < tol, (rows, n, e, em)
         off += rows * n
 
 

@@ -17,7 +17,7 @@ import pytest
 
 from oracle import core
 from oracle import noise as onoise
-from tests.helpers import rel_err
+from tests.helpers import max_rel_err, rel_err
 
 pytestmark = pytest.mark.gpu
 FULL = os.environ.get("MK_FULL_PRECISION") == "1"
@@ -33,8 +33,8 @@ def _stage_errors(P, k=0):
     tod.noise.rot_v.exec(d, rot)
     e_rot = rel_err(rot.cpu().numpy(), v @ d64)
     onz = onoise.NoiseSmoothedSVD(d64, fwhm=tod.noise.fwhm, u_in=v.T.copy())  # same basis as the device model
-    e_an = rel_err(tod.noise.apply_noise(d).cpu().numpy(), onz.apply_noise(d64))
-    return e_rot, e_an
+    got, want = tod.noise.apply_noise(d).cpu().numpy(), onz.apply_noise(d64)
+    return e_rot, rel_err(got, want), max_rel_err(got, want)
 
 
 def _tod2map_error(P):
@@ -95,12 +95,13 @@ def test_precision_workload_errors():
     else:
         P = build_problem(3, shapes=[(1024, 32768)])
         iters = 20
-    e_rot, e_an = _stage_errors(P)
+    e_rot, e_an, e_an_max = _stage_errors(P)
     e_t2m = _tod2map_error(P)
     e_rhs, e_map, rel = _maps_after(P, iters)
     rep = {"workload": "precision: %d TODs of %s, 20%% of detectors in a rank-5 common mode (10x white)" % (
         len(P.shapes), "x".join(map(str, P.shapes[0]))), "iterations": iters,
-        "rotation_rel_err": e_rot, "apply_noise_rel_err": e_an, "tod2map_rel_err": e_t2m,
+        "rotation_rel_err": e_rot, "apply_noise_rel_err": e_an, "apply_noise_max_rel_err": e_an_max,
+        "tod2map_rel_err": e_t2m,
         "rhs_rel_err_vs_fp64_device": e_rhs, "map_rel_err_vs_fp64_device": e_map, "fp32_rel_res": rel}
     print(json.dumps(rep))
     out = os.environ.get("MK_PRECISION_REPORT")
@@ -108,7 +109,10 @@ def test_precision_workload_errors():
         with open(out, "w") as f:
             json.dump(rep, f, indent=1)
     assert e_rot < 1e-6       # six split-BF16 products: ~FP32 rounding
-    assert e_an < 1e-5        # the apply_noise tolerance
+    assert e_an < 1e-5 and e_an_max < 1e-5  # the apply_noise tolerance, norm-wise and max-elementwise
     assert e_t2m < 1e-5
     assert e_rhs < 1e-5
-    assert e_map < 1e-2       # CG iterates amplify the FP32 operator error; the value is stated in the report
+    # CG iterates amplify the FP32 operator error: measured 8.4e-6 on the
+    # shortened workload (20 iterations) and 2.6e-4 on the full one (100
+    # iterations); the bounds are 5x those
+    assert e_map < (1.3e-3 if FULL else 4.2e-5)
