@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -652,7 +653,19 @@ int t2m_exec(const mk_t2m_plan* pl, TM* map, const TD* dat, int accumulate, void
   const int tile_px = pl->tile_x * pl->tile_y;
   const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
   const size_t lds = 4 * (size_t)tile_px * sizeof(T);
-  MK_REQUIRE(lds <= 65536, MK_ERR_ARG, "tile too large for LDS with this precision");
+  // four wave tiles of the accumulation type; above 64 KiB (FP64 sums of
+  // tiles over 2048 pixels) the kernel is allowed the rest of gfx950's LDS
+  constexpr size_t kMaxT2MLds = 160 * 1024;
+  MK_REQUIRE(lds <= kMaxT2MLds, MK_ERR_ARG, "tile of %d pixels too large for LDS with this precision", tile_px);
+  if (lds > 65536) {
+    static std::once_flag once;
+    std::call_once(once, [] {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
+    });
+  }
   // FP64 partial tiles whenever the plan holds them (8-byte partial
   // elements); an FP64 accumulation with 4-byte partials rounds each item's
   // sum once

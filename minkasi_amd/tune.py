@@ -217,12 +217,13 @@ class Tuner:
         dat.fill_(1.0)
         out = torch.zeros(nx * ny, dtype=torch.float32, device=ipix.device)
 
+        # timed with the FP32 sums the PCG iteration's A.x uses
         def timed(plan):
-            plan.exec(out, dat, False)
+            plan.exec(out, dat, False, acc64=False)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(reps):
-                plan.exec(out, dat, False)
+                plan.exec(out, dat, False, acc64=False)
             e1.record()
             e1.synchronize()
             return e0.elapsed_time(e1) / reps

@@ -22,10 +22,12 @@ def _fp64_build(**kw):
     return todvec, m, otods
 
 
-def test_t2m_fp64_sums_match_bincount():
+@pytest.mark.parametrize("tile", [(32, 32), (64, 64)])
+def test_t2m_fp64_sums_match_bincount(tile):
     """FP32 samples binned with FP64 sums: equal to the FP64 bincount of the
     same samples to summation-order rounding (1e-13), into an FP64 map and
-    (rounded once) into an FP32 map; bitwise repeatable."""
+    (rounded once) into an FP32 map; bitwise repeatable.  The 64 x 64 tile
+    needs 128 KiB of LDS for its four FP64 wave tiles."""
     import torch
     from minkasi_amd.lib.plans import T2MPlan
     rng = np.random.default_rng(11)
@@ -38,7 +40,7 @@ def test_t2m_fp64_sums_match_bincount():
     ref = np.bincount(ipix, weights=dat.astype(np.float64), minlength=nx * ny)
     ip_d = torch.from_numpy(ipix).cuda()
     d_d = torch.from_numpy(dat).cuda()
-    plan = T2MPlan(ip_d, nx, ny, tile=(32, 32))
+    plan = T2MPlan(ip_d, nx, ny, tile=tile)
     m64 = torch.zeros(nx * ny, dtype=torch.float64, device="cuda")
     plan.exec(m64, d_d, False)
     got = m64.cpu().numpy()
