@@ -245,6 +245,17 @@ int mk_pcg_set_r(float *r, const double *b, const double *ax, int64_t n,
 int64_t mk_dct1_workspace_bytes(int64_t nrow, int64_t n, int elem_bytes);
 int mk_dct1_f64(const double *x, double *y, int64_t nrow, int64_t n,
                 void *work, int64_t work_bytes, void *stream);
+/* mk_dct1_f64 without a caller workspace: no counterpart in the reference
+ * (its FFTW plans allocate, minkasi/lib/mkfftw.c:174-212).  For callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc): the workspace is a static 64 MiB device array of the library,
+ * one per GPU, so nrow and n must satisfy
+ * mk_dct1_workspace_bytes(nrow, n, 8) <= 64 MiB (else MK_ERR_ARG with the
+ * sizes; e.g. 30 rows at n = 65536).  Allocates nothing; synchronises the
+ * stream once, as mk_dct1_f64.  Calls that overlap on one GPU share that
+ * array and race: serialise them (one stream, or the caller's lock). */
+int mk_dct1_f64_noalloc(const double *x, double *y, int64_t nrow, int64_t n,
+                        void *stream);
 int mk_dct1_f32(const float *x, float *y, int64_t nrow, int64_t n,
                 void *work, int64_t work_bytes, void *stream);
 

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -1815,6 +1816,52 @@ int dct1_impl(const TIO* x, TIO* y, int64_t nrow, int64_t n, void* work, int64_t
   return mk::check_launch("dct1");
 }
 }  // namespace
+
+// Static workspace of mk_dct1_f64_noalloc: the Bluestein workspace (segment
+// descriptors, chirp spectrum, nrow x P complex FP64 rows) for callers that
+// cannot allocate device memory.  64 MiB per device that runs the library,
+// held until the process ends.  Host code never uses this name as an
+// address: it asks hipGetSymbolAddress for each device's copy once, with
+// that device current, and passes that pointer on.
+constexpr int64_t kScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (looked up on its first use).
+int scratch_ptr(void** out) {
+  int dev = -1;
+  MK_HIP(hipGetDevice(&dev));
+  MK_REQUIRE(dev >= 0 && dev < kMaxDevices, MK_ERR_ARG, "device index %d outside 0..%d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MK_REQUIRE(p != nullptr && ((uintptr_t)p & 255) == 0, MK_ERR_HIP, "scratch address lookup gave %p", p);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MK_OK;
+}
+}  // namespace
+
+extern "C" int mk_dct1_f64_noalloc(const double* x, double* y, int64_t nrow, int64_t n, void* stream) {
+  MK_REQUIRE(nrow >= 0 && n >= 2, MK_ERR_ARG, "DCT-I needs nrow >= 0 and n >= 2 (got %lld, %lld)", (long long)nrow,
+             (long long)n);
+  if (nrow == 0) return MK_OK;
+  const int64_t need = mk_dct1_workspace_bytes(nrow, n, 8);
+  MK_REQUIRE(need > 0 && need <= kScratchBytes, MK_ERR_ARG,
+             "mk_dct1_f64_noalloc: %lld rows of %lld points need %lld B of workspace, more than its %lld B static "
+             "scratch; use mk_dct1_f64 with a caller workspace (or fewer rows per call)",
+             (long long)nrow, (long long)n, (long long)need, (long long)kScratchBytes);
+  void* ws = nullptr;
+  int r = scratch_ptr(&ws);
+  if (r) return r;
+  return dct1_impl<double, double>(x, y, nrow, n, ws, kScratchBytes, stream);
+}
 
 extern "C" int mk_dct1_f64(const double* x, double* y, int64_t nrow, int64_t n, void* work, int64_t work_bytes,
                            void* stream) {

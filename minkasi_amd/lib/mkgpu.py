@@ -85,6 +85,7 @@ _SIGS = {
     "mk_dct1_workspace_bytes": ([I64, I64, I], I64),
     "mk_dct1_f64": ([P, P, I64, I64, P, I64, P], I),
     "mk_dct1_f32": ([P, P, I64, I64, P, I64, P], I),
+    "mk_dct1_f64_noalloc": ([P, P, I64, I64, P], I),
     "mk_filt_plan_create": ([I, P, I, ctypes.POINTER(P)], I),
     "mk_filt_plan_set_weights": ([P, P, P], I),
     "mk_filt_plan_set_halve": ([P, I], I),
