@@ -306,20 +306,27 @@ class SkyMap:
         self._write_fits(fname, self.map)
 
     def _write_fits(self, fname, plane):
-        data = plane.to(torch.float64).cpu().numpy().T.copy()
-        cards = [("SIMPLE", True), ("BITPIX", -64), ("NAXIS", 2), ("NAXIS1", data.shape[1]),
-                 ("NAXIS2", data.shape[0])] + self.wcs.to_header()
-        fname = os.path.abspath(fname)
-        os.makedirs(os.path.dirname(fname), exist_ok=True)
-        with open(fname, "wb") as f:
-            f.write(_fits_header(cards))
-            raw = data.astype(">f8").tobytes()
-            f.write(raw)
-            pad = (-len(raw)) % 2880
-            f.write(b"\0" * pad)
+        write_map_fits(fname, plane.to(torch.float64).cpu().numpy(), self.wcs)
 
     def plot(self, plot_info=None):
         raise NotImplementedError("plotting is outside the mapmaking inner loop")
+
+
+def write_map_fits(fname, m, wcs):
+    """A FITS primary HDU holding the (nx, ny) map m transposed, as FP64
+    (BITPIX -64, big-endian), with the WCS cards (skymap.py:705-722 writes
+    fits.PrimaryHDU(map.T, header=wcs.to_header()))."""
+    data = np.asarray(m, dtype=np.float64).T.copy()
+    cards = [("SIMPLE", True), ("BITPIX", -64), ("NAXIS", 2), ("NAXIS1", data.shape[1]),
+             ("NAXIS2", data.shape[0])] + wcs.to_header()
+    fname = os.path.abspath(fname)
+    os.makedirs(os.path.dirname(fname), exist_ok=True)
+    with open(fname, "wb") as f:
+        f.write(_fits_header(cards))
+        raw = data.astype(">f8").tobytes()
+        f.write(raw)
+        pad = (-len(raw)) % 2880
+        f.write(b"\0" * pad)
 
 
 def _fits_header(cards):

@@ -176,13 +176,21 @@ class CarWCS:
         return px, py
 
     def to_header(self):
-        """FITS header cards for the map (SkyMap.write)."""
+        """FITS header cards for the map (SkyMap.write): the cards, in the
+        order, that astropy's WCS.to_header() gives for this CAR WCS (the
+        reference writes its maps with it, skymap.py:705-722): the axis
+        count, reference pixel, increments and units, projection types,
+        reference value, the native pole (LONPOLE/LATPOLE), MJDREF 0 and
+        RADESYS ICRS (astropy's defaults when no equinox or date is set)."""
         return [
+            ("WCSAXES", 2),
+            ("CRPIX1", float(self.crpix[0])), ("CRPIX2", float(self.crpix[1])),
+            ("CDELT1", float(self.cdelt[0])), ("CDELT2", float(self.cdelt[1])),
+            ("CUNIT1", "deg"), ("CUNIT2", "deg"),
             ("CTYPE1", self.ctype[0]), ("CTYPE2", self.ctype[1]),
             ("CRVAL1", float(self.crval[0])), ("CRVAL2", float(self.crval[1])),
-            ("CDELT1", float(self.cdelt[0])), ("CDELT2", float(self.cdelt[1])),
-            ("CRPIX1", float(self.crpix[0])), ("CRPIX2", float(self.crpix[1])),
             ("LONPOLE", float(self.lonpole)), ("LATPOLE", float(self.latpole)),
+            ("MJDREF", 0.0), ("RADESYS", "ICRS"),
         ]
 
     def __deepcopy__(self, memo):

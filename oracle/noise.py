@@ -7,8 +7,12 @@ Functions, what they follow, and what pins them (tests/test_oracle_noise.py):
                        DCT-I^2 = 2(n-1) I and the delta/ones known answers.
   smooth_many_vecs  -- minkasi/tools/smooth.py:49-78, restated line by line;
                        pinned by the closed form (circular convolution of the
-                       even extensions) on small n and by smoothing a constant
-                       (must return the constant).
+                       even extensions) on small n, by smoothing a constant
+                       (must return the constant), and, for the Gaussian's
+                       width, by a single-bin spectrum that must come back as
+                       exp(-j^2 / 2 sigma^2) with sigma = fwhm / sqrt(8 ln 2),
+                       half maximum at +-fwhm/2 and peak 1/(sqrt(2 pi) sigma)
+                       (test_smoothing_width_closed_form).
   NoiseSmoothedSVD  -- minkasi/mapmaking/noise.py:584-772 with
                        svd(full_matrices=False) (the full_matrices=True call in
                        the reference builds an n x n factor and cannot run at

@@ -165,3 +165,13 @@ def test_noise_prewhiten_matches_oracle(env):
     assert np.allclose(g.noisevec, ref.noisevec, rtol=1e-12)
     x = rng.standard_normal((ndet, n))
     assert rel_err(g.apply_noise(device.to_device(x)).cpu().numpy(), ref.apply_noise(x)) < 1e-8
+
+
+@pytest.mark.parametrize("fwhm", [10.0, 50.0])
+def test_device_smoothing_width_closed_form(env, fwhm):
+    """The device smoothing (mk_smooth_rows_f64) against the same closed form
+    as the oracle: a single-bin spectrum comes back as the Gaussian of the
+    stated FWHM with peak 1/(sqrt(2 pi) sigma)."""
+    from minkasi_amd.tools.smooth import smooth_many_vecs
+    from tests.test_oracle_noise import _single_bin_closed_form
+    _single_bin_closed_form(smooth_many_vecs, fwhm)

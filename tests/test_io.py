@@ -182,3 +182,32 @@ def test_cut_blacklist(capsys):
     names = ["/d/b.fits", "/d/a.fits", "/e/c.fits"]
     assert cut_blacklist(names, ["x/a.fits", "zz.fits"]) == ["/d/b.fits", "/e/c.fits"]
     assert "deleted  1  bad files." in capsys.readouterr().out
+
+
+def test_skymap_write_matches_astropy_fixture(tmp_path):
+    """SkyMap.write's file against the one astropy writes for the same CAR
+    WCS and map (fits.PrimaryHDU(map.T, header=wcs.to_header()),
+    minkasi/maps/skymap.py:705-722; tests/golden/make_skymap_fixture.py):
+    the same cards in the same order with the same values (astropy prints
+    14 significant digits), and the same data, bit for bit."""
+    from minkasi_amd.maps.skymap import write_map_fits
+    from minkasi_amd.maps.wcs import get_wcs
+    from minkasi_amd.tods.io import fits_open
+    fx = np.load(os.path.join(G, "skymap_write.npz"))
+    w = get_wcs(list(fx["lims"]), float(fx["pixsize"]))
+    fname = str(tmp_path / "map.fits")
+    write_map_fits(fname, fx["map"], w)
+    ours, ref = fits_open(fname)[0], fits_open(os.path.join(G, "skymap_write.fits"))[0]
+    keys = [c[:8].strip() for c in ref.header.cards]
+    assert keys == list(fx["keys"])
+    assert [c[:8].strip() for c in ours.header.cards] == keys
+    for k in keys:
+        a, b = ours.header[k], ref.header[k]
+        if isinstance(b, float):
+            assert abs(a - b) <= 1e-12 * max(abs(b), 1e-300), (k, a, b)
+        else:
+            assert a == b and type(a) == type(b), (k, a, b)
+    assert ours.data.dtype == ref.data.dtype and np.array_equal(ours.data, ref.data)
+    assert np.array_equal(ours.data.T, fx["map"])
+    with open(fname, "rb") as f:
+        assert len(f.read()) % 2880 == 0

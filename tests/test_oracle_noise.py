@@ -82,3 +82,34 @@ def test_get_det_weights_formula():
     w = nz.get_det_weights()
     ref = 2 * np.einsum("ij,j,ji->i", nz.vT, nz.mywt.sum(axis=1), nz.v)
     assert np.allclose(w, ref)
+
+
+def _single_bin_closed_form(smooth, fwhm, n=2001, k0=1000):
+    """smooth(delta at bin k0) against the Gaussian the FWHM defines, with no
+    use of the taps' construction: sigma = fwhm / sqrt(8 ln 2), the profile
+    exp(-j^2 / (2 sigma^2)) around k0 with half maximum at +-fwhm/2 bins,
+    and peak 1/(sqrt(2 pi) sigma), since the r2r-normalised taps sum to the
+    Gaussian's integral (Poisson summation: relative error about
+    2 exp(-2 pi^2 sigma^2), below 1e-100 here).  The mirror image of the
+    bin lies 2(n-1) - 2 k0 = 2000 bins away and does not contribute."""
+    P = np.zeros((1, n))
+    P[0, k0] = 1.0
+    s = np.asarray(smooth(P, fwhm))[0]
+    sig = fwhm / np.sqrt(8 * np.log(2))
+    j = np.arange(-int(3 * fwhm), int(3 * fwhm) + 1)
+    prof = s[k0 + j] / s[k0]
+    assert np.allclose(prof, np.exp(-0.5 * (j / sig) ** 2), rtol=0, atol=1e-13)
+    h = int(round(fwhm / 2))
+    assert abs(s[k0 + h] / s[k0] - 0.5) < 1e-13 and abs(s[k0 - h] / s[k0] - 0.5) < 1e-13
+    assert abs(s[k0] * np.sqrt(2 * np.pi) * sig - 1.0) < 1e-12
+    # a width used as sigma (or sigma as the width) would fail both checks
+    wrong = np.exp(-0.5 * (h / fwhm) ** 2)
+    assert abs(wrong - 0.5) > 0.3
+
+
+@pytest.mark.parametrize("fwhm", [10.0, 50.0])
+def test_smoothing_width_closed_form(fwhm):
+    """Pins the FWHM -> sigma conversion of smooth_many_vecs
+    (minkasi/tools/smooth.py:66-78) by a closed form, independently of
+    gaussian_taps."""
+    _single_bin_closed_form(noise.smooth_many_vecs, fwhm)
