@@ -76,44 +76,64 @@ def stage_bytes(P):
     }
 
 
-def cpu_baseline(P, budget_s=20.0):
-    """FP64 oracle (OpenMP projections, numpy GEMM, scipy DCT-I) on one TOD of
-    the workload's median size; scaled to PCG iterations/s of the full job."""
+def _largest_prime_factor(k):
+    f, big = 2, 1
+    while f * f <= k:
+        while k % f == 0:
+            big, k = f, k // f
+        f += 1
+    return max(big, k) if k > 1 else big
+
+
+def cpu_baseline(P, budget_s=25.0, max_tods=12):
+    """FP64 oracle (OpenMP projections, numpy GEMM, scipy DCT-I) timed on
+    TODs of the workload drawn at random (fixed seed) until the budget is
+    spent: the per-sample A.x cost of that sample, scaled to PCG
+    iterations/s of the full job.  The DCT-I cost depends on how n - 1
+    factors (scipy falls back to Bluestein for large primes), so the sample
+    spans several lengths rather than one median TOD."""
     import torch
-    from oracle import core, noise as onoise, solve
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    from oracle import noise as onoise, solve
+    # the box gives one GPU 16 host cores (OMP_NUM_THREADS is set to that
+    # share there); a larger affinity mask here uses at most that many
+    cores = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
     os.environ["OMP_NUM_THREADS"] = str(cores)
     import scipy.fft
-    shapes = sorted(P.shapes, key=lambda s: s[0] * s[1])
-    ndet, n = shapes[len(shapes) // 2]
-    k = [i for i, s in enumerate(P.shapes) if s == (ndet, n)][0]
-    tod = P.todvec.tods[k]
-    d = tod.get_data()
-    d = d.cpu().numpy().astype(np.float64) if isinstance(d, torch.Tensor) else np.asarray(d, np.float64)
-    ipix = tod.info[P.map.tag].cpu().numpy().reshape(ndet, n)
+    order = np.random.default_rng(0).permutation(len(P.shapes))
+    x = np.random.default_rng(0).standard_normal((P.map.nx, P.map.ny))
+    t_start = time.time()
+    picked, t_ax, s_ax, t_init = [], 0.0, 0, 0.0
     with scipy.fft.set_workers(cores):
-        t0 = time.time()
-        nz = onoise.NoiseSmoothedSVD(d, fwhm=50.0)
-        t_init = time.time() - t0
-        otod = solve.OTod(d, ipix, nz)
-        x = np.random.default_rng(0).standard_normal((P.map.nx, P.map.ny))
-        reps, t_ax = 0, 0.0
-        while reps < 1 or (t_ax < budget_s and reps < 5):
+        for k in order:
+            if picked and (time.time() - t_start > budget_s or len(picked) >= max_tods):
+                break
+            ndet, n = P.shapes[k]
+            tod = P.todvec.tods[k]
+            d = tod.get_data()
+            d = d.cpu().numpy().astype(np.float64) if isinstance(d, torch.Tensor) else np.asarray(d, np.float64)
+            ipix = tod.info[P.map.tag].cpu().numpy().reshape(ndet, n)
+            t0 = time.time()
+            nz = onoise.NoiseSmoothedSVD(d, fwhm=50.0)
+            t_init += time.time() - t0
+            otod = solve.OTod(d, ipix, nz)
             t0 = time.time()
             solve.todvec_dot([otod], x)
             t_ax += time.time() - t0
-            reps += 1
-    per_ax = t_ax / reps
-    scale = P.nsamp / float(ndet * n)
+            s_ax += ndet * n
+            picked.append((ndet, n, _largest_prime_factor(n - 1)))
+            del d, otod, nz
+    per_sample = t_ax / s_ax
     return {
-        "value": 1.0 / (per_ax * scale),
+        "value": 1.0 / (per_sample * P.nsamp),
         "unit": "PCG iter/s",
         "cores": cores,
         "kind": "port",
-        "sample": "FP64 oracle A.x on one TOD of %d x %d samples (%d reps, %.2f s each; noise init %.1f s), "
-                  "scaled by total/sample samples = %.1f; FFTW unavailable, scipy DCT-I" % (
-                      ndet, n, reps, per_ax, t_init, scale),
-        "msamp_per_s": ndet * n / per_ax / 1e6,
+        "sample": "FP64 oracle A.x on %d of the %d TODs drawn at random (seed 0; ndet x n [largest prime of n-1]: %s), "
+                  "%.3g samples in %.2f s, scaled to the job's %.3g samples; noise init %.1f s untimed; %d threads, "
+                  "the box's host-core share for one GPU; FFTW unavailable, scipy DCT-I" % (
+                      len(picked), len(P.shapes), ", ".join("%dx%d [%d]" % p for p in picked), s_ax, t_ax,
+                      P.nsamp, t_init, cores),
+        "msamp_per_s": s_ax / t_ax / 1e6,
     }
 
 
