@@ -113,6 +113,39 @@ int mk_map_to_tiles_f64(const double *map, int32_t nx, int32_t ny,
                         int32_t tile_x, int32_t tile_y, double *tiled,
                         void *stream);
 
+/* Coded pointing for the tiled gather: map2tod_omp (minkasi.c:131-144)
+ * reading about 1 byte of index per sample instead of 4.  The arena's
+ * samples are cut into blocks of 256; a block keeps its first sample's pixel
+ * (x0, y0) and per sample the offset from it, as int8 dx with dy = 0
+ * (type 1, 1 B), int8 (dx, dy) (type 2, 2 B) or packed (x << 16 | y)
+ * (type 3, 4 B).  Map sides must be below 32768.
+ *  mk_pix_code_blocks: number of blocks, ceil(nsamp / 256).
+ *  mk_pix_code_size: per block, payload size in 256-byte units (1, 2 or 4)
+ *      into units (device int32, nblocks); indices outside [0, nx*ny) are
+ *      counted into *dev_nbad (device int64, caller-zeroed).
+ *  mk_pix_code_write: with off_units the exclusive prefix sum of units
+ *      (device int32, total below 2^30), writes the headers (hdr: device,
+ *      8 bytes per block) and the payload (256-byte aligned, 256 * total
+ *      units bytes).
+ *  mk_map2tod_coded_*: dat (+)= tiled[index(x, y)] over the arena, with
+ *      tiled the map copy of mk_map_to_tiles_* for tile_x x tile_y tiles
+ *      (powers of two); equal bit for bit to mk_map2tod_* on the map and
+ *      the original indices.  dat 4-element aligned. */
+int64_t mk_pix_code_blocks(int64_t nsamp);
+int mk_pix_code_size(const int32_t *ipix, int64_t nsamp, int32_t nx, int32_t ny,
+                     int32_t *units, int64_t *dev_nbad, void *stream);
+int mk_pix_code_write(const int32_t *ipix, int64_t nsamp, int32_t ny,
+                      const int32_t *off_units, const int32_t *units,
+                      void *hdr, void *payload, void *stream);
+int mk_map2tod_coded_f32(float *dat, const float *tiled, int64_t nsamp,
+                         const void *hdr, const void *payload, int32_t ny,
+                         int32_t tile_x, int32_t tile_y, int do_add,
+                         void *stream);
+int mk_map2tod_coded_f64(double *dat, const double *tiled, int64_t nsamp,
+                         const void *hdr, const void *payload, int32_t ny,
+                         int32_t tile_x, int32_t tile_y, int do_add,
+                         void *stream);
+
 /* tod2map with global atomics: map[ipix[i]] += dat[i].
  * Counterpart of tod2map_atomic (minkasi.c:59-66); order-nondeterministic
  * like the reference's tod2map_omp/tod2map_atomic.  Kept as the

@@ -751,3 +751,271 @@ extern "C" int mk_map_to_tiles_f64(const double* map, int32_t nx, int32_t ny, in
                                    double* tiled, void* stream) {
   return map_tiles_impl<double>(map, nx, ny, tile_x, tile_y, tiled, stream);
 }
+
+// ---------------------------------------------------------------------------
+// Coded pointing for the tiled gather.  A raster TOD moves a fraction of a
+// pixel per sample, so the 256 samples of a block span a few dozen map
+// columns and usually one map row.  Each block keeps its first sample's
+// pixel (x0, y0) and, per sample, the offset from it:
+//   type 1: dx as int8, every dy = 0            1 byte per sample
+//   type 2: (dx, dy) as two int8                2 bytes per sample
+//   type 3: (x << 16 | y) as int32              4 bytes per sample (fallback)
+// Header per block: int2 {x0 << 16 | y0, payload offset in 256-byte units
+// | type << 30}.  The gather decodes (x, y), forms the tiled index with
+// shifts and masks (tile sides are powers of two) and reads the tiled map:
+// about 1 byte of index traffic per sample instead of 4, bit-exact.
+
+namespace {
+constexpr int kCodeBlock = 256;  // samples per block: one wave, 4 per lane
+
+__device__ __forceinline__ bool code_fits8(int v) { return v >= -128 && v <= 127; }
+
+// One wave per block: payload size (in 256-byte units) and bad indices.
+__global__ void __launch_bounds__(256) pix_code_size_kernel(const int32_t* __restrict__ ipix, int64_t n, int32_t nx,
+                                                            int32_t ny, int64_t nblocks,
+                                                            int32_t* __restrict__ units, int64_t* nbad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t npix = (int64_t)nx * ny;
+  for (int64_t b = wave; b < nblocks; b += nwaves) {
+    const int64_t s0 = b * kCodeBlock;
+    int x[4], y[4];
+    bool ok[4], bad = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int64_t i = s0 + 4 * lane + k;
+      ok[k] = i < n;
+      const int32_t p = ok[k] ? ipix[i] : 0;
+      const bool b_ = ok[k] && (p < 0 || (int64_t)p >= npix);
+      bad |= b_;
+      ok[k] = ok[k] && !b_;
+      x[k] = ok[k] ? p / ny : 0;
+      y[k] = ok[k] ? p - x[k] * ny : 0;
+    }
+    mk::count_bad(bad, nbad);
+    const int x0 = __shfl(x[0], 0, 64), y0 = __shfl(y[0], 0, 64);
+    bool a = true, c = true;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int dx = x[k] - x0, dy = y[k] - y0;
+      a = a && (!ok[k] || (dy == 0 && code_fits8(dx)));
+      c = c && (!ok[k] || (code_fits8(dx) && code_fits8(dy)));
+    }
+    const bool all_a = __ballot(!a) == 0, all_c = __ballot(!c) == 0;
+    if (lane == 0) units[b] = all_a ? 1 : (all_c ? 2 : 4);
+  }
+}
+
+__global__ void __launch_bounds__(256) pix_code_write_kernel(const int32_t* __restrict__ ipix, int64_t n, int32_t ny,
+                                                             int64_t nblocks, const int32_t* __restrict__ off_units,
+                                                             const int32_t* __restrict__ units,
+                                                             int2* __restrict__ hdr, uint8_t* __restrict__ payload) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t b = wave; b < nblocks; b += nwaves) {
+    const int64_t s0 = b * kCodeBlock;
+    int x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int64_t i = s0 + 4 * lane + k;
+      const int32_t p = i < n ? ipix[i] : ipix[s0];  // past the end: a copy of the first (never gathered)
+      x[k] = p / ny;
+      y[k] = p - x[k] * ny;
+    }
+    const int x0 = __shfl(x[0], 0, 64), y0 = __shfl(y[0], 0, 64);
+    const int u = units[b];
+    const int type = u == 1 ? 1 : (u == 2 ? 2 : 3);
+    uint8_t* base = payload + (int64_t)off_units[b] * 256;
+    if (type == 1) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) w |= (uint32_t)(uint8_t)(int8_t)(x[k] - x0) << (8 * k);
+      reinterpret_cast<uint32_t*>(base)[lane] = w;
+    } else if (type == 2) {
+      uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        w0 |= ((uint32_t)(uint8_t)(int8_t)(x[k] - x0) | (uint32_t)(uint8_t)(int8_t)(y[k] - y0) << 8) << (16 * k);
+        w1 |= ((uint32_t)(uint8_t)(int8_t)(x[k + 2] - x0) | (uint32_t)(uint8_t)(int8_t)(y[k + 2] - y0) << 8)
+              << (16 * k);
+      }
+      reinterpret_cast<uint2*>(base)[lane] = make_uint2(w0, w1);
+    } else {
+      int4 w;
+      w.x = (x[0] << 16) | y[0];
+      w.y = (x[1] << 16) | y[1];
+      w.z = (x[2] << 16) | y[2];
+      w.w = (x[3] << 16) | y[3];
+      reinterpret_cast<int4*>(base)[lane] = w;
+    }
+    if (lane == 0) hdr[b] = make_int2((x0 << 16) | y0, off_units[b] | (type << 30));
+  }
+}
+
+// Tiled index of pixel (x, y) for tx = 1 << ltx, ty = 1 << lty.
+__device__ __forceinline__ int32_t tiled_index(int x, int y, int ltx, int lty, int32_t nty) {
+  return (((x >> ltx) * nty + (y >> lty)) << (ltx + lty)) + ((x & ((1 << ltx) - 1)) << lty) + (y & ((1 << lty) - 1));
+}
+
+// Decodes the four pixels of quad q (samples 4q..4q+3) of block q / 64.
+__device__ __forceinline__ void decode_quad(const int2* __restrict__ hdr, const uint8_t* __restrict__ payload,
+                                            int64_t q, int* x, int* y) {
+  const int2 h = hdr[q >> 6];
+  const int x0 = (int)((uint32_t)h.x >> 16), y0 = h.x & 0xffff;
+  const int type = (int)((uint32_t)h.y >> 30);
+  const uint8_t* base = payload + (int64_t)(h.y & 0x3fffffff) * 256;
+  const int l = (int)(q & 63);
+  if (type == 1) {
+    const uint32_t w = reinterpret_cast<const uint32_t*>(base)[l];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      x[k] = x0 + (int)(int8_t)(w >> (8 * k));
+      y[k] = y0;
+    }
+  } else if (type == 2) {
+    const uint2 w = reinterpret_cast<const uint2*>(base)[l];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      x[k] = x0 + (int)(int8_t)(w.x >> (16 * k));
+      y[k] = y0 + (int)(int8_t)(w.x >> (16 * k + 8));
+      x[k + 2] = x0 + (int)(int8_t)(w.y >> (16 * k));
+      y[k + 2] = y0 + (int)(int8_t)(w.y >> (16 * k + 8));
+    }
+  } else {
+    const int4 w = reinterpret_cast<const int4*>(base)[l];
+    x[0] = (int)((uint32_t)w.x >> 16); y[0] = w.x & 0xffff;
+    x[1] = (int)((uint32_t)w.y >> 16); y[1] = w.y & 0xffff;
+    x[2] = (int)((uint32_t)w.z >> 16); y[2] = w.z & 0xffff;
+    x[3] = (int)((uint32_t)w.w >> 16); y[3] = w.w & 0xffff;
+  }
+}
+
+// dat (+)= tiled[index(x, y)] over the arena, 4 samples per thread, U quads
+// per loop trip with their header / payload loads and gathers in flight.
+// The stride is a multiple of 64 quads, so a wave always covers whole
+// blocks (one header, one block type per wave and step).
+template <typename T>
+__global__ void __launch_bounds__(256) map2tod_coded_kernel(T* __restrict__ dat, const T* __restrict__ tiled,
+                                                            int64_t n, const int2* __restrict__ hdr,
+                                                            const uint8_t* __restrict__ payload, int ltx, int lty,
+                                                            int32_t nty, int do_add) {
+  constexpr int U = 4;  // measured: 2.74 ms at 4, 2.92 ms at 8 (1.9e9 samples)
+  const int64_t n4 = n >> 2;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  Quad<T>* d4 = reinterpret_cast<Quad<T>*>(dat);
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q + (U - 1) * stride < n4; q += U * stride) {
+    int x[U][4], y[U][4];
+#pragma unroll
+    for (int u = 0; u < U; u++) decode_quad(hdr, payload, q + u * stride, x[u], y[u]);
+    Quad<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[u].v[k] = tiled[tiled_index(x[u][k], y[u][k], ltx, lty, nty)];
+    if (do_add) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const Quad<T> o = d4[q + u * stride];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[u].v[k] += o.v[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) d4[q + u * stride] = v[u];
+  }
+  for (; q < n4; q += stride) {
+    int x[4], y[4];
+    decode_quad(hdr, payload, q, x, y);
+    Quad<T> v;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v.v[k] = tiled[tiled_index(x[k], y[k], ltx, lty, nty)];
+    if (do_add) {
+      const Quad<T> o = d4[q];
+#pragma unroll
+      for (int k = 0; k < 4; k++) v.v[k] += o.v[k];
+    }
+    d4[q] = v;
+  }
+  // the last n % 4 samples: one thread decodes their quad
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 3)) {
+    int x[4], y[4];
+    decode_quad(hdr, payload, n4, x, y);
+    for (int k = 0; k < (int)(n & 3); k++) {
+      const T v = tiled[tiled_index(x[k], y[k], ltx, lty, nty)];
+      dat[4 * n4 + k] = do_add ? dat[4 * n4 + k] + v : v;
+    }
+  }
+}
+
+int log2_exact(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  return (1 << l) == v ? l : -1;
+}
+
+template <typename T>
+int map2tod_coded_impl(T* dat, const T* tiled, int64_t nsamp, const int2* hdr, const uint8_t* payload, int32_t ny,
+                       int32_t tile_x, int32_t tile_y, int do_add, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(dat && tiled && hdr && payload, MK_ERR_ARG, "null pointer");
+  const int ltx = log2_exact(tile_x), lty = log2_exact(tile_y);
+  MK_REQUIRE(ltx >= 0 && lty >= 0 && tile_x * tile_y <= 1024, MK_ERR_ARG,
+             "coded gather needs power-of-two tile sides (got %d x %d)", tile_x, tile_y);
+  MK_REQUIRE(ny > 0 && ny < 32768, MK_ERR_ARG, "bad ny");
+  MK_REQUIRE(((uintptr_t)dat % (4 * sizeof(T))) == 0 && ((uintptr_t)payload & 255) == 0, MK_ERR_ARG,
+             "coded gather: dat must be 4-element aligned and the payload 256-byte aligned");
+  const int32_t nty = (ny + tile_y - 1) / tile_y;
+  // a stride of 64-quad multiples: 256-thread workgroups
+  const unsigned grid = mk::grid_for(mk::cdiv(nsamp, 16), 256, 8192);
+  map2tod_coded_kernel<T><<<grid, 256, 0, mk::as_stream(stream)>>>(dat, tiled, nsamp, hdr, payload, ltx, lty, nty,
+                                                                    do_add);
+  return mk::check_launch("map2tod_coded_kernel");
+}
+}  // namespace
+
+extern "C" int64_t mk_pix_code_blocks(int64_t nsamp) { return nsamp <= 0 ? 0 : mk::cdiv(nsamp, kCodeBlock); }
+
+extern "C" int mk_pix_code_size(const int32_t* ipix, int64_t nsamp, int32_t nx, int32_t ny, int32_t* units,
+                                int64_t* dev_nbad, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  MK_REQUIRE(nx > 0 && ny > 0 && nx < 32768 && ny < 32768, MK_ERR_ARG,
+             "coded pointing needs map sides below 32768 (got %d x %d)", nx, ny);
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(ipix && units && dev_nbad, MK_ERR_ARG, "null pointer");
+  const int64_t nb = mk::cdiv(nsamp, kCodeBlock);
+  const unsigned grid = (unsigned)std::min<int64_t>(mk::cdiv(nb, 4), 256 * 64);
+  pix_code_size_kernel<<<grid, 256, 0, mk::as_stream(stream)>>>(ipix, nsamp, nx, ny, nb, units, dev_nbad);
+  return mk::check_launch("pix_code_size_kernel");
+}
+
+extern "C" int mk_pix_code_write(const int32_t* ipix, int64_t nsamp, int32_t ny, const int32_t* off_units,
+                                 const int32_t* units, void* hdr, void* payload, void* stream) {
+  MK_REQUIRE(nsamp >= 0, MK_ERR_ARG, "nsamp < 0");
+  MK_REQUIRE(ny > 0 && ny < 32768, MK_ERR_ARG, "bad ny");
+  if (nsamp == 0) return MK_OK;
+  MK_REQUIRE(ipix && off_units && units && hdr && payload, MK_ERR_ARG, "null pointer");
+  MK_REQUIRE(((uintptr_t)payload & 255) == 0 && ((uintptr_t)hdr & 7) == 0, MK_ERR_ARG,
+             "payload must be 256-byte aligned, headers 8-byte aligned");
+  const int64_t nb = mk::cdiv(nsamp, kCodeBlock);
+  const unsigned grid = (unsigned)std::min<int64_t>(mk::cdiv(nb, 4), 256 * 64);
+  pix_code_write_kernel<<<grid, 256, 0, mk::as_stream(stream)>>>(ipix, nsamp, ny, nb, off_units, units, (int2*)hdr,
+                                                                  (uint8_t*)payload);
+  return mk::check_launch("pix_code_write_kernel");
+}
+
+extern "C" int mk_map2tod_coded_f32(float* dat, const float* tiled, int64_t nsamp, const void* hdr,
+                                    const void* payload, int32_t ny, int32_t tile_x, int32_t tile_y, int do_add,
+                                    void* stream) {
+  return map2tod_coded_impl<float>(dat, tiled, nsamp, (const int2*)hdr, (const uint8_t*)payload, ny, tile_x, tile_y,
+                                   do_add, stream);
+}
+extern "C" int mk_map2tod_coded_f64(double* dat, const double* tiled, int64_t nsamp, const void* hdr,
+                                    const void* payload, int32_t ny, int32_t tile_x, int32_t tile_y, int do_add,
+                                    void* stream) {
+  return map2tod_coded_impl<double>(dat, tiled, nsamp, (const int2*)hdr, (const uint8_t*)payload, ny, tile_x,
+                                    tile_y, do_add, stream);
+}

@@ -129,6 +129,11 @@ _SIGS = {
     "mk_pix_tile_remap": ([P, I64, I32, I32, I32, I32, P, P, P], I),
     "mk_map_to_tiles_f32": ([P, I32, I32, I32, I32, P, P], I),
     "mk_map_to_tiles_f64": ([P, I32, I32, I32, I32, P, P], I),
+    "mk_pix_code_blocks": ([I64], I64),
+    "mk_pix_code_size": ([P, I64, I32, I32, P, P, P], I),
+    "mk_pix_code_write": ([P, I64, I32, P, P, P, P, P], I),
+    "mk_map2tod_coded_f32": ([P, P, I64, P, P, I32, I32, I32, I, P], I),
+    "mk_map2tod_coded_f64": ([P, P, I64, P, P, I32, I32, I32, I, P], I),
     "mk_defl_plan_create": ([I, P, P, P], I),
     "mk_defl_plan_exec": ([P, P, P, P], I),
     "mk_defl_plan_addback": ([P, P, P], I),

@@ -240,3 +240,60 @@ class DeflPlan(_Handle):
 
     def addback(self, y):
         mkgpu.check("mk_defl_plan_addback", self.h, dev.ptr(y, torch.float32, "y"), dev.stream_ptr())
+
+
+class PixCode:
+    """Coded pointing of an ipix arena for the tiled gather
+    (mk_pix_code_*, mk_map2tod_coded_*): blocks of 256 samples holding their
+    first pixel and per-sample offsets in 1, 2 or 4 bytes."""
+
+    @staticmethod
+    def sizes(ipix, nx, ny):
+        """(units per block, total payload bytes) for ipix; raises on
+        indices outside the nx x ny map."""
+        n = int(ipix.numel())
+        nb = int(mkgpu.lib.mk_pix_code_blocks(n))
+        units = torch.empty(max(nb, 1), dtype=torch.int32, device=ipix.device)
+        nbad = torch.zeros(1, dtype=torch.int64, device=ipix.device)
+        if n:
+            mkgpu.check("mk_pix_code_size", dev.ptr(ipix, torch.int32, "ipix"), n, int(nx), int(ny),
+                        units.data_ptr(), nbad.data_ptr(), dev.stream_ptr())
+        nb_bad = int(nbad.item())
+        if nb_bad:
+            raise ValueError("%d samples have pixel indices outside the %dx%d map" % (nb_bad, nx, ny))
+        u = units[:nb].cpu().numpy().astype(np.int64)
+        return units[:nb], int(u.sum()) * 256 + 8 * nb
+
+    def __init__(self, ipix, nx, ny, units=None):
+        n = int(ipix.numel())
+        d = ipix.device
+        if units is None:
+            units, _ = PixCode.sizes(ipix, nx, ny)
+        nb = int(units.numel())
+        u = units.cpu().numpy().astype(np.int64)
+        off = np.zeros(max(nb, 1), dtype=np.int64)
+        if nb > 1:
+            off[1:nb] = np.cumsum(u)[:-1]
+        total = int(u.sum())
+        if total >= 1 << 30:
+            raise ValueError("pointing of %d samples too large for one coded stream" % n)
+        self.nsamp, self.nx, self.ny, self.nblocks = n, int(nx), int(ny), nb
+        self.hdr = torch.empty(2 * max(nb, 1), dtype=torch.int32, device=d)
+        self.payload = torch.empty(max(total, 1) * 256, dtype=torch.uint8, device=d)
+        if n:
+            off_d = torch.from_numpy(off[:nb].astype(np.int32)).to(d)
+            mkgpu.check("mk_pix_code_write", dev.ptr(ipix, torch.int32, "ipix"), n, self.ny, off_d.data_ptr(),
+                        units.data_ptr(), self.hdr.data_ptr(), self.payload.data_ptr(), dev.stream_ptr())
+            torch.cuda.current_stream().synchronize()  # off_d is freed on return
+        self.nbytes = self.hdr.numel() * 4 + self.payload.numel()
+        self.blocks_by_type = {t: int((u == k).sum()) for t, k in ((1, 1), (2, 2), (3, 4))}
+
+    def gather(self, dat, tiled, tile, do_add=False):
+        """dat (+)= tiled map gathered at the coded pixels (tile: its (tx, ty))."""
+        name = {torch.float32: "mk_map2tod_coded_f32", torch.float64: "mk_map2tod_coded_f64"}[dat.dtype]
+        if dat.numel() != self.nsamp or tiled.dtype != dat.dtype:
+            raise ValueError("coded gather: %d samples of %s for a code of %d" % (dat.numel(), dat.dtype,
+                                                                                 self.nsamp))
+        mkgpu.check(name, dev.ptr(dat, name="dat"), tiled.data_ptr(), self.nsamp, self.hdr.data_ptr(),
+                    self.payload.data_ptr(), self.ny, int(tile[0]), int(tile[1]), int(bool(do_add)),
+                    dev.stream_ptr())

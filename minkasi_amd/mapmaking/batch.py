@@ -23,7 +23,7 @@ import torch
 
 from .. import device as dev
 from ..lib import mkgpu
-from ..lib.plans import DeflPlan, FiltPlan, RotPlan, T2MPlan
+from ..lib.plans import DeflPlan, FiltPlan, PixCode, RotPlan, T2MPlan
 from .map2tod import map2tod
 from .noise import NoiseSmoothedSVD
 
@@ -147,13 +147,17 @@ class BatchedDot:
         else:
             self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)
         work = [int(tod.noise.filt32.work_bytes) for tod in self.tods]
-        self.tiled, cap = self._memory_plan(work, off, tiled_gather, filter_work_cap)
-        self.ipix_g = None
+        # the tiled gather reads coded pointing (about 1 byte per sample; its
+        # exact size from one sizing pass) and a tiled copy of the map
+        units, code_bytes = (PixCode.sizes(self.ipix[:off], self.nx, self.ny) if off and tiled_gather is not False
+                             else (None, 0))
+        self.tiled, cap = self._memory_plan(work, off, tiled_gather, filter_work_cap, index_bytes=code_bytes)
+        self.code = None
         self.map_t = None
         self.gather_tile = None
-        if self.tiled:
-            # gather indices into a tiled copy of the map (see _set_gather_tile)
-            self.ipix_g = torch.empty(max(off, 4), dtype=torch.int32, device=d)
+        if self.tiled and off:
+            self.code = PixCode(self.ipix[:off], self.nx, self.ny, units=units)
+        del units
         if self.tiled and tuner is not None and off > 0:
             sig = tune.ipix_signature(self.ipix[:off], self.nx, self.ny)
             x = torch.randn(self.nx, self.ny, device=d)
@@ -201,27 +205,30 @@ class BatchedDot:
         self.t2m_acc64 = False
 
     @staticmethod
-    def _memory_plan(work, nsamp, tiled_gather, filter_work_cap, free=None):
+    def _memory_plan(work, nsamp, tiled_gather, filter_work_cap, free=None, index_bytes=None):
         """(tiled gather?, filter work cap in bytes) from the device memory
         free once the tod2map plan is built: less a reserve, it goes to the
-        tiled gather's index copy (4 bytes per sample) and the filter work
-        buffer, and the index copy is given up first when the filter would
-        otherwise get less than MIN_FILTER_WORK (or its largest TOD's work)."""
+        tiled gather's index structure (index_bytes, at most 4 bytes per
+        sample) and the filter work buffer, and the tiled gather is given up
+        first when the filter would otherwise get less than MIN_FILTER_WORK
+        (or its largest TOD's work)."""
         if not work:
             return bool(tiled_gather is not False), 0
         if free is None:
             free = int(torch.cuda.mem_get_info()[0])
+        if index_bytes is None:
+            index_bytes = 4 * nsamp
         left = free - MEM_RESERVE
         total, top = sum(work), max(work)
         if tiled_gather is None:
-            tiled_gather = left - 4 * nsamp >= max(top, min(total, MIN_FILTER_WORK))
+            tiled_gather = left - index_bytes >= max(top, min(total, MIN_FILTER_WORK))
         if tiled_gather:
-            left -= 4 * nsamp
+            left -= index_bytes
         if left < top:
             raise MemoryError(
                 "batched operator over %.3g samples: %.1f GB of device memory free after the tod2map plan, not "
                 "enough for %sa filter work buffer of %.1f GB and a %.1f GB reserve" % (
-                    nsamp, free / 1e9, "tiled gather indices (%.1f GB), " % (4 * nsamp / 1e9) if tiled_gather else "",
+                    nsamp, free / 1e9, "tiled gather indices (%.1f GB), " % (index_bytes / 1e9) if tiled_gather else "",
                     top / 1e9, MEM_RESERVE / 1e9))
         cap = left if filter_work_cap is None else min(left, int(filter_work_cap))
         return bool(tiled_gather), cap
@@ -235,24 +242,16 @@ class BatchedDot:
         """Device bytes held by the operator's own buffers and plans."""
         n = self.nsamp
         t = self.t2m.info()
-        return {"work_tods": 8 * n, "ipix": 4 * n, "tiled_gather_ipix": 4 * n if self.tiled else 0,
+        return {"work_tods": 8 * n, "ipix": 4 * n, "tiled_gather_ipix": self.code.nbytes if self.code else 0,
                 "tod2map_plan": t["plan_bytes"] + t["partial_bytes"], "filter_work": int(self.filter_work.numel()),
                 "filter_groups": len(self.filts)}
 
     def _set_gather_tile(self, tile):
-        """Rewrite the gather indices for tx x ty map tiles and size the
-        tiled map buffer."""
+        """Size the tiled map buffer for tx x ty map tiles (powers of two);
+        the coded pointing does not depend on the tile."""
         if tile == self.gather_tile:
             return
         tx, ty = int(tile[0]), int(tile[1])
-        n = self.nsamp
-        if n:
-            nbad = torch.zeros(1, dtype=torch.int64, device=self.ipix.device)
-            mkgpu.check("mk_pix_tile_remap", self.ipix.data_ptr(), n, self.nx, self.ny, tx, ty,
-                        self.ipix_g.data_ptr(), nbad.data_ptr(), dev.stream_ptr())
-            nb = int(nbad.item())
-            if nb:
-                raise ValueError("%d samples have pixel indices outside the %dx%d map" % (nb, self.nx, self.ny))
         size = int(mkgpu.lib.mk_map_tiled_size(self.nx, self.ny, tx, ty))
         if self.map_t is None or self.map_t.numel() != size:
             self.map_t = torch.zeros(size, dtype=torch.float32, device=self.ipix.device)
@@ -272,6 +271,8 @@ class BatchedDot:
     def _map2tod(self, x, a):
         """a = x.flat[ipix] over the arena, gathered from the tiled copy (or
         from the row-major map when memory is short)."""
+        if not self.nsamp:
+            return
         if not self.tiled:
             # the arena's indices were checked against nx * ny at construction
             map2tod(a, x.reshape(-1), self.ipix[:self.nsamp], do_add=False, check=False)
@@ -279,7 +280,7 @@ class BatchedDot:
         tx, ty = self.gather_tile
         mkgpu.check("mk_map_to_tiles_f32", dev.ptr(x, torch.float32, "map"), self.nx, self.ny, tx, ty,
                     self.map_t.data_ptr(), dev.stream_ptr())
-        map2tod(a, self.map_t, self.ipix_g[:self.nsamp], do_add=False, check=False)
+        self.code.gather(a, self.map_t, self.gather_tile)
 
     @staticmethod
     def _time(fn, reps=3):

@@ -207,7 +207,7 @@ def test_short_memory_operator_matches_default():
     work = [t.noise.filt32.work_bytes for t in todvec.tods]
     small = BatchedDot(todvec, m, tiled_gather=False, filter_work_cap=2 * max(work))
     assert op.tiled and len(op.filts) == 1
-    assert not small.tiled and small.ipix_g is None and len(small.filts) >= 3
+    assert not small.tiled and small.code is None and len(small.filts) >= 3
     assert sorted(i for g in small.filter_groups for i in g) == list(range(5))
     assert small.filter_work.numel() <= 2 * max(work)
     x = torch.randn((m.nx, m.ny), device=m.map.device)

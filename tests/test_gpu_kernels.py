@@ -375,3 +375,87 @@ def test_batched_gather_equals_rowmajor_gather(env):
         a = torch.empty(n, dtype=torch.float32, device=x.device)
         op._map2tod(x, a)
         assert torch.equal(a, want), t
+
+
+def _decode_pix_code(hdr, payload, n, ny):
+    """numpy decoder of the coded pointing (block header {x0 << 16 | y0,
+    offset in 256-byte units | type << 30}; types 1, 2, 3)."""
+    hdr = hdr.reshape(-1, 2).view(np.uint32)
+    out = np.empty(n, dtype=np.int64)
+    for b in range((n + 255) // 256):
+        x0, y0 = int(hdr[b, 0] >> 16), int(hdr[b, 0] & 0xFFFF)
+        typ, off = int(hdr[b, 1] >> 30), int(hdr[b, 1] & 0x3FFFFFFF) * 256
+        m = min(256, n - 256 * b)
+        if typ == 1:
+            dx = payload[off:off + 256].view(np.int8)[:m].astype(np.int64)
+            x, y = x0 + dx, np.full(m, y0)
+        elif typ == 2:
+            d = payload[off:off + 512].view(np.int8).reshape(256, 2)[:m].astype(np.int64)
+            x, y = x0 + d[:, 0], y0 + d[:, 1]
+        else:
+            w = payload[off:off + 1024].view(np.uint32)[:m].astype(np.int64)
+            x, y = w >> 16, w & 0xFFFF
+        out[256 * b:256 * b + m] = x * ny + y
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_coded_pointing_decodes_and_gathers_bit_exact(dtype):
+    """Coded pointing (1, 2 or 4 bytes per sample in 256-sample blocks): the
+    stream decodes to the original indices, and the coded gather from the
+    tiled map equals map2tod on the original indices bit for bit, for every
+    gather tile, with and without do_add.  The input mixes raster rows (type
+    1), blocks that step in dec (type 2), blocks that jump across the map
+    (type 3) and a last block of n % 256 = 237 samples (n % 4 = 1)."""
+    import torch
+    from minkasi_amd import device
+    from minkasi_amd.lib import mkgpu
+    from minkasi_amd.lib.plans import PixCode
+    from minkasi_amd.mapmaking.batch import GATHER_TILES
+    rng = np.random.default_rng(21)
+    nx, ny = 1027, 771
+    rows, n_row = 7, 40001
+    x = (np.arange(n_row) * 0.3).astype(np.int64) % 600 + 100
+    parts = []
+    for r in range(rows):
+        y = np.full(n_row, 50 + 13 * r)
+        y[21000:] += 10                                    # a dec step inside a block (no x wrap there)
+        parts.append(x * ny + y)
+    jump = rng.integers(0, nx * ny, 3 * 256)                # blocks of scattered pixels
+    ipix = np.concatenate(parts + [jump]).astype(np.int32)
+    ipix = np.concatenate([ipix, ipix[:(237 - ipix.size % 256) % 256]])
+    n = ipix.size
+    assert n % 256 == 237 and n % 4 == 1
+    ip_d = torch.from_numpy(ipix).cuda()
+    code = PixCode(ip_d, nx, ny)
+    assert code.blocks_by_type[1] > 0 and code.blocks_by_type[2] > 0 and code.blocks_by_type[3] > 0
+    print("coded pointing: %.3f bytes per sample, blocks by type %s" % (code.nbytes / n, code.blocks_by_type))
+    assert code.nbytes < 1.5 * n
+    dec = _decode_pix_code(code.hdr.cpu().numpy(), code.payload.cpu().numpy(), n, ny)
+    assert np.array_equal(dec, ipix)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    m = torch.from_numpy(rng.standard_normal((nx, ny)).astype(dtype)).cuda()
+    ref = torch.empty(n, dtype=tdt, device="cuda")
+    mkgpu.check("mk_map2tod_f32" if dtype == np.float32 else "mk_map2tod_f64", ref.data_ptr(), m.data_ptr(), n,
+                ip_d.data_ptr(), 0, device.stream_ptr())
+    base = torch.from_numpy(rng.standard_normal(n).astype(dtype)).cuda()
+    ref_add = base + ref
+    for tx, ty in GATHER_TILES:
+        tiled = torch.zeros(int(mkgpu.lib.mk_map_tiled_size(nx, ny, tx, ty)), dtype=tdt, device="cuda")
+        mkgpu.check("mk_map_to_tiles_f32" if dtype == np.float32 else "mk_map_to_tiles_f64", m.data_ptr(), nx, ny,
+                    tx, ty, tiled.data_ptr(), device.stream_ptr())
+        got = torch.full((n,), float("nan"), dtype=tdt, device="cuda")
+        code.gather(got, tiled, (tx, ty))
+        assert torch.equal(got, ref), (tx, ty)
+        acc = base.clone()
+        code.gather(acc, tiled, (tx, ty), do_add=True)
+        assert torch.equal(acc, ref_add), (tx, ty)
+
+
+def test_coded_pointing_rejects_bad_indices():
+    import torch
+    from minkasi_amd.lib.plans import PixCode
+    ipix = np.arange(1000, dtype=np.int32)
+    ipix[777] = 64 * 64
+    with pytest.raises(ValueError):
+        PixCode(torch.from_numpy(ipix).cuda(), 64, 64)
