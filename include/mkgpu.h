@@ -185,6 +185,21 @@ int mk_t2m_plan_exec_f64(const mk_t2m_plan *plan, double *map,
 int mk_t2m_plan_exec_f32_acc64(const mk_t2m_plan *plan, void *map,
                                int map_f64, const float *dat, int accumulate,
                                void *stream);
+/* Deterministic polarised binning through the same plan: map (nx*ny x
+ * ncomp, component-minor, PolMap's layout) (+)= the per-sample terms of mode
+ * (MK_POL_*: QU -> d cos, d sin; IQU -> d, d cos, d sin; QU_PRECON ->
+ * d cos^2, d sin^2, d sin cos; IQU_PRECON -> d, d cos, d sin, d cos^2,
+ * d sin cos, d sin^2, with cos, sin of twogamma), one binning pass per
+ * component in the plan's fixed order: bitwise repeatable, where the
+ * reference's tod2map_{qu,iqu}[_precon]_simple (minkasi.c:178-258) sum
+ * serially and mk_tod2polmap_atomic_* sums in arrival order.  FP32 samples
+ * are summed in FP64. */
+int mk_t2m_plan_exec_pol_f32(const mk_t2m_plan *plan, float *map,
+                             const float *dat, const float *twogamma, int mode,
+                             int accumulate, void *stream);
+int mk_t2m_plan_exec_pol_f64(const mk_t2m_plan *plan, double *map,
+                             const double *dat, const double *twogamma,
+                             int mode, int accumulate, void *stream);
 /* info[0..7] = nsamp, nitems, ntiles_touched, tile_x, tile_y, plan bytes,
  * partial-buffer bytes, max samples per item. */
 int mk_t2m_plan_info(const mk_t2m_plan *plan, int64_t *info);

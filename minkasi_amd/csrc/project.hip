@@ -382,14 +382,38 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
 // group, in group order: the result does not depend on timing).
 constexpr int kT2MUnroll = 16;
 
+// Weight of polarisation component comp of mode POL (MK_POL_*) for a sample
+// at angle 2 gamma with sin s and cos c (minkasi.c:178-258: QU -> (c, s),
+// IQU -> (1, c, s), QU_PRECON -> (c^2, s^2, s c), IQU_PRECON -> (1, c, s,
+// c^2, s c, s^2)).
+template <int POL, typename T>
+__device__ __forceinline__ T pol_weight(int comp, T s, T c) {
+  if (POL == MK_POL_QU) return comp == 0 ? c : s;
+  if (POL == MK_POL_IQU) return comp == 0 ? T(1) : (comp == 1 ? c : s);
+  if (POL == MK_POL_QU_PRECON) return comp == 0 ? c * c : (comp == 1 ? s * s : s * c);
+  switch (comp) {
+    case 0: return T(1);
+    case 1: return c;
+    case 2: return s;
+    case 3: return c * c;
+    case 4: return s * c;
+    default: return s * s;
+  }
+}
+__device__ __forceinline__ void pol_sincos(float a, float* s, float* c) { sincosf(a, s, c); }
+__device__ __forceinline__ void pol_sincos(double a, double* s, double* c) { sincos(a, s, c); }
+
 // TD: sample type; T: accumulation type (FP64 for FP32 samples keeps the
 // per-pixel sums of many cancelling samples at FP64 accuracy); P: partials.
-template <typename TD, typename T, typename P>
+// POL: 0 bins the samples; MK_POL_* bins sample * pol_weight(comp, 2 gamma)
+// with 2 gamma read from twogamma at the same entry.
+template <typename TD, typename T, typename P, int POL = 0>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
                                                         const uint32_t* __restrict__ entry,
                                                         const uint16_t* __restrict__ lpix,
                                                         const TD* __restrict__ dat, int tile_px,
-                                                        P* __restrict__ partial) {
+                                                        P* __restrict__ partial,
+                                                        const TD* __restrict__ twogamma = nullptr, int comp = 0) {
   extern __shared__ unsigned char smem_raw[];
   T* tiles = reinterpret_cast<T*>(smem_raw);  // [4][tile_px]
   const T2MItem it = items[blockIdx.x];
@@ -418,7 +442,12 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
     }
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
-      const T x = (T)base[e[j]];
+      T x = (T)base[e[j]];
+      if constexpr (POL != 0) {
+        TD sn, cs;
+        pol_sincos(twogamma[it.seg_base + e[j]], &sn, &cs);
+        x = (T)(base[e[j]] * pol_weight<POL, TD>(comp, sn, cs));
+      }
       v[j] = valid[j] ? x : T(0);
       lp[j] = valid[j] ? lp[j] : -1 - lane;  // distinct sentinels never match
     }
@@ -462,7 +491,8 @@ template <typename T, typename P, typename TM>
 __global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restrict__ tile_items,
                                                         const P* __restrict__ partial, int tile_x, int tile_y,
                                                         int ntile_y, int nx, int ny, TM* __restrict__ map,
-                                                        int accumulate, int blocks_per_tile) {
+                                                        int accumulate, int blocks_per_tile, int ncomp = 1,
+                                                        int comp = 0) {
   const int tile = blockIdx.x / blocks_per_tile;
   const int tile_px = tile_x * tile_y;
   const int p0 = ((blockIdx.x % blocks_per_tile) * 256 + threadIdx.x) * 4;
@@ -490,7 +520,7 @@ __global__ void __launch_bounds__(256) t2m_tiles_kernel(const int32_t* __restric
   for (int k = 0; k < 4; k++) {
     const int y = y0 + k;
     if (y >= ny) break;
-    TM* m = map + (int64_t)x * ny + y;
+    TM* m = map + ((int64_t)x * ny + y) * ncomp + comp;
     *m = (TM)(accumulate ? (T)*m + acc[k] : acc[k]);
   }
 }
@@ -708,6 +738,87 @@ extern "C" int mk_t2m_plan_exec_f32_acc64(const mk_t2m_plan* plan, void* map, in
   if (map_f64) return t2m_exec<float, double, double>(plan, (double*)map, dat, accumulate, stream);
   return t2m_exec<float, double, float>(plan, (float*)map, dat, accumulate, stream);
 }
+namespace {
+// map (npix x ncomp, component-minor as PolMap holds it) (+)= the per-sample
+// terms of mode, component by component through the plan: each component
+// is one deterministic binning of sample * weight.  FP32 samples are
+// summed in FP64.
+template <typename TD, int POL>
+int t2m_exec_pol_mode(const mk_t2m_plan* pl, TD* map, const TD* dat, const TD* twogamma, int accumulate,
+                      hipStream_t stream) {
+  typedef double T;
+  const int ncomp = POL == MK_POL_QU ? 2 : (POL == MK_POL_IQU_PRECON ? 6 : 3);
+  const int tile_px = pl->tile_x * pl->tile_y;
+  const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
+  const size_t lds = 4 * (size_t)tile_px * sizeof(T);
+  constexpr size_t kMaxT2MLds = 160 * 1024;
+  MK_REQUIRE(lds <= kMaxT2MLds, MK_ERR_ARG, "tile of %d pixels too large for LDS with FP64 sums", tile_px);
+  const bool p64 = pl->partial_elem_bytes == 8;
+  MK_REQUIRE(p64 || sizeof(TD) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
+  if (lds > 65536) {
+    static std::once_flag once;
+    std::call_once(once, [] {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, double, POL>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, float, POL>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
+    });
+  }
+  const int bpt = (int)mk::cdiv(tile_px / 4, 256);
+  const unsigned nblk = (unsigned)(ntiles * bpt);
+  for (int c = 0; c < ncomp; c++) {
+    if (pl->nitems > 0) {
+      if (p64)
+        t2m_items_kernel<TD, T, double, POL><<<(unsigned)pl->nitems, 256, lds, stream>>>(
+            pl->items, pl->entry, pl->lpix, dat, tile_px, (double*)pl->partial, twogamma, c);
+      else
+        t2m_items_kernel<TD, T, float, POL><<<(unsigned)pl->nitems, 256, lds, stream>>>(
+            pl->items, pl->entry, pl->lpix, dat, tile_px, (float*)pl->partial, twogamma, c);
+      int st = mk::check_launch("t2m_items_kernel (pol)");
+      if (st) return st;
+    }
+    if (p64)
+      t2m_tiles_kernel<T, double, TD><<<nblk, 256, 0, stream>>>(pl->tile_items, (const double*)pl->partial,
+                                                                pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx, pl->ny,
+                                                                map, accumulate, bpt, ncomp, c);
+    else
+      t2m_tiles_kernel<T, float, TD><<<nblk, 256, 0, stream>>>(pl->tile_items, (const float*)pl->partial,
+                                                               pl->tile_x, pl->tile_y, pl->ntile_y, pl->nx, pl->ny,
+                                                               map, accumulate, bpt, ncomp, c);
+    int st = mk::check_launch("t2m_tiles_kernel (pol)");
+    if (st) return st;
+  }
+  return MK_OK;
+}
+
+template <typename TD>
+int t2m_exec_pol(const mk_t2m_plan* pl, TD* map, const TD* dat, const TD* twogamma, int mode, int accumulate,
+                 void* stream_v) {
+  MK_REQUIRE(pl && map, MK_ERR_ARG, "null plan/map");
+  MK_REQUIRE(pl->nsamp == 0 || (dat && twogamma), MK_ERR_ARG, "null dat/twogamma");
+  hipStream_t st = mk::as_stream(stream_v);
+  switch (mode) {
+    case MK_POL_QU: return t2m_exec_pol_mode<TD, MK_POL_QU>(pl, map, dat, twogamma, accumulate, st);
+    case MK_POL_IQU: return t2m_exec_pol_mode<TD, MK_POL_IQU>(pl, map, dat, twogamma, accumulate, st);
+    case MK_POL_QU_PRECON: return t2m_exec_pol_mode<TD, MK_POL_QU_PRECON>(pl, map, dat, twogamma, accumulate, st);
+    case MK_POL_IQU_PRECON:
+      return t2m_exec_pol_mode<TD, MK_POL_IQU_PRECON>(pl, map, dat, twogamma, accumulate, st);
+    default: break;
+  }
+  mk::set_error("unknown polarisation mode %d", mode);
+  return MK_ERR_ARG;
+}
+}  // namespace
+
+extern "C" int mk_t2m_plan_exec_pol_f32(const mk_t2m_plan* plan, float* map, const float* dat,
+                                        const float* twogamma, int mode, int accumulate, void* stream) {
+  return t2m_exec_pol<float>(plan, map, dat, twogamma, mode, accumulate, stream);
+}
+extern "C" int mk_t2m_plan_exec_pol_f64(const mk_t2m_plan* plan, double* map, const double* dat,
+                                        const double* twogamma, int mode, int accumulate, void* stream) {
+  return t2m_exec_pol<double>(plan, map, dat, twogamma, mode, accumulate, stream);
+}
+
 extern "C" int mk_t2m_plan_info(const mk_t2m_plan* pl, int64_t* info) {
   MK_REQUIRE(pl && info, MK_ERR_ARG, "null pointer");
   info[0] = pl->nsamp;

@@ -66,6 +66,8 @@ _SIGS = {
     "mk_t2m_plan_exec_f32": ([P, P, P, I, P], I),
     "mk_t2m_plan_exec_f64": ([P, P, P, I, P], I),
     "mk_t2m_plan_exec_f32_acc64": ([P, P, I, P, I, P], I),
+    "mk_t2m_plan_exec_pol_f32": ([P, P, P, P, I, I, P], I),
+    "mk_t2m_plan_exec_pol_f64": ([P, P, P, P, I, I, P], I),
     "mk_t2m_plan_info": ([P, P], I),
     "mk_t2m_plan_destroy": ([P], I),
     "mk_dot_f32": ([P, P, I64, P, P, P], I),

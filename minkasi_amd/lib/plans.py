@@ -88,6 +88,24 @@ class T2MPlan(_Handle):
             mkgpu.check("mk_t2m_plan_exec_f32", self.h, mptr, dptr, acc, dev.stream_ptr())
 
 
+    def exec_pol(self, map_t, dat, twogamma, mode, accumulate):
+        """Polarised map (nx*ny x ncomp) (+)= the per-sample terms of mode
+        (mkgpu.POL_*) through this plan: deterministic, FP64 sums."""
+        if dat.numel() != self.nsamp or twogamma.numel() != self.nsamp:
+            raise ValueError("TOD / twogamma sizes differ from the plan's %d samples" % self.nsamp)
+        if dat.dtype != map_t.dtype or twogamma.dtype != map_t.dtype:
+            raise TypeError("map, TOD and twogamma must share a dtype")
+        ncomp = {mkgpu.POL_QU: 2, mkgpu.POL_IQU: 3, mkgpu.POL_QU_PRECON: 3, mkgpu.POL_IQU_PRECON: 6}[mode]
+        if map_t.numel() != self.nx * self.ny * ncomp:
+            raise ValueError("map holds %d values, the plan's %dx%d map with %d components needs %d" % (
+                map_t.numel(), self.nx, self.ny, ncomp, self.nx * self.ny * ncomp))
+        name = {torch.float32: "mk_t2m_plan_exec_pol_f32", torch.float64: "mk_t2m_plan_exec_pol_f64"}[dat.dtype]
+        dptr = dev.ptr(dat, name="dat") if self.nsamp else dev.ptr(map_t)
+        gptr = dev.ptr(twogamma, name="twogamma") if self.nsamp else dev.ptr(map_t)
+        mkgpu.check(name, self.h, dev.ptr(map_t, name="map"), dptr, gptr, int(mode), int(bool(accumulate)),
+                    dev.stream_ptr())
+
+
 class FiltPlan(_Handle):
     """Batched C W C filter (noise filter core)."""
 

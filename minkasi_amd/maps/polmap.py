@@ -6,7 +6,7 @@ code, polmap.py:88-195 and :386-462).  With npol > 1 the map is an
 (nx, ny, npol) device tensor, the reference's layout.  Its projections run
 in libmkgpu:
   map2tod   mk_polmap2tod_*            (QU, IQU)
-  tod2map   mk_tod2polmap_atomic_*     (QU, IQU, QU_PRECON, IQU_PRECON)
+  tod2map   mk_t2m_plan_exec_pol_*     (QU, IQU, QU_PRECON, IQU_PRECON; deterministic)
   invert    mk_polmap_invert_*         (per-pixel pseudo-inverse)
   __mul__   mk_polmap_precon_mul_*     (preconditioner block times Stokes vector)
 Each sample's polarisation angle comes from tod.info['twogamma_saved'], as
@@ -14,8 +14,10 @@ in the reference.  It is copied to the device once, in the map's precision,
 and kept in tod.info.  With npol == 1 ('I') a PolMap behaves as a SkyMap.
 
 Differences from the reference:
-  * tod2map with npol > 1 sums with device atomics, so the order of the sums
-    is not fixed (the reference sums serially).
+  * tod2map with npol > 1 bins each component through the TOD's
+    deterministic tod2map plan (FP64 sums, fixed order); the reference sums
+    serially, in sample order.  tod2map_method = "atomic" keeps the device
+    atomics (order not fixed).
   * invert does not print the determinant range.
 """
 import copy
@@ -109,7 +111,12 @@ class PolMap(SkyMap):
         if not do_add:
             self.clear()
         d = dev.to_device(dat, self.map.dtype)
-        tod2polmap(self.map, d, self.poltag, self._twogamma(tod), self.get_pix(tod), npix=self._npix())
+        ipix = self.get_pix(tod)
+        if self.tod2map_method == "atomic":
+            tod2polmap(self.map, d, self.poltag, self._twogamma(tod), ipix, npix=self._npix())
+        else:
+            # deterministic: the TOD's cached tod2map plan, one pass per component
+            self._t2m_plan(tod, ipix).exec_pol(self.map, d, self._twogamma(tod), _MODE[self.poltag], True)
         if self.purge_pixellization:
             tod.clear_saved_pix(self.tag)
 

@@ -512,3 +512,55 @@ def test_gapfill_eig_matches_oracle(env, host):
     want = d.copy()
     want.flat[imap] += out.map.cpu().numpy()
     assert rel_err(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_polmap_deterministic_binning_matches_oracle(env, dtype):
+    """Polarised binning through the tod2map plan (one pass per component,
+    FP64 sums): equal to the oracle's serial sums (oracle/oracle.c, pinned
+    against the reference's own minkasi.c) to summation-order rounding for
+    every mode, bitwise repeatable, and PolMap.tod2map uses it."""
+    torch, device, mkgpu = env
+    import minkasi_amd as mk
+    from minkasi_amd.lib.plans import T2MPlan
+    rng = np.random.default_rng(6)
+    nx, ny = 31, 27
+    npix = nx * ny
+    ndet, n = 10, 7001
+    ipix, tg, dat = _pol_inputs(rng, ndet, n, npix, dtype)
+    tg_c = tg.astype(dtype).astype(np.float64)
+    dat_c = dat.astype(dtype).astype(np.float64)
+    d_ipix = device.to_device(ipix)
+    d_tg, d_dat = device.to_device(tg.astype(dtype)), device.to_device(dat.astype(dtype))
+    plan = T2MPlan(d_ipix.reshape(-1), nx, ny, tile=(16, 8))
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    for mode in (1, 2, 3, 4):
+        nc = core.POL_NCOMP[mode]
+        ref = np.zeros(npix * nc)
+        core.tod2polmap(ref, dat_c, tg_c, ipix, mode)
+        got = torch.zeros(npix * nc, dtype=tdt, device=device.device())
+        plan.exec_pol(got, d_dat.reshape(-1), d_tg.reshape(-1), mode, False)
+        again = torch.full_like(got, 9.0)
+        plan.exec_pol(again, d_dat.reshape(-1), d_tg.reshape(-1), mode, False)
+        assert torch.equal(got, again), mode
+        e = np.abs(got.cpu().numpy() - ref).max() / np.abs(ref).max()
+        # FP32: the angles' FP32 sin/cos and one rounding of each product; FP64: order only
+        assert e < (1e-13 if dtype == np.float64 else 2e-6), (mode, e)
+        acc = torch.ones_like(got)
+        plan.exec_pol(acc, d_dat.reshape(-1), d_tg.reshape(-1), mode, True)
+        assert np.abs(acc.cpu().numpy() - 1.0 - got.cpu().numpy()).max() <= 1e-6 * np.abs(ref).max()
+    # PolMap.tod2map: the deterministic path by default, the atomic one on request
+    lims = [1.0, 1.001, 0.5, 0.501]
+    pm = mk.PolMap(lims, 1e-5, poltag="IQU", nx=nx, ny=ny, dtype=tdt)
+    tod = mk.Tod({"dat_calib": dat.astype(dtype), "twogamma_saved": tg.astype(dtype), "dx": np.zeros((ndet, n)),
+                  "dy": np.zeros((ndet, n))})
+    tod.info[pm.tag] = d_ipix
+    pm.tod2map(tod, tod.get_data())
+    ref = np.zeros(npix * 3)
+    core.tod2polmap(ref, dat_c, tg_c, ipix, 2)
+    first = pm.map.clone()
+    pm.clear()
+    pm.tod2map(tod, tod.get_data())
+    assert torch.equal(pm.map, first)
+    assert np.abs(first.cpu().numpy().ravel() - ref).max() / np.abs(ref).max() < (1e-13 if dtype == np.float64
+                                                                                   else 2e-6)
