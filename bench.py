@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3: setup + steps only")
     ap.add_argument("--weights", default=os.path.join(HERE, "minkasi_amd", "learned", "weights_c4.npz"),
                     help="trained learned-preconditioner weights")
+    ap.add_argument("--chain-weights", default=os.path.join(HERE, "minkasi_amd", "learned", "weights_c4_chain.npz"),
+                    help="weights of the 4-layer chain network, solved and reported beside --weights ('' to skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration) and exit")
     return ap.parse_args()
@@ -373,6 +375,18 @@ def main():
             res["iteration_ratio"] = round(jac["iters"] / max(lrn["iters"], 1), 3)
             res["map_distance"] = map_distance(P, lrn["x"], jac["x"])
             log("learned to tol:", json.dumps(res["learned"]), "speedup %.2fx" % res["speedup_wall"])
+            # the network the workload names (4 dilated 3x3 layers in series,
+            # 16 channels), reported beside the shipped multilevel net
+            if args.chain_weights and os.path.exists(args.chain_weights):
+                params, arch, spec, meta = load(args.chain_weights)
+                lc = LearnedPrecon(P.hits, params, arch, spec)
+                ch = solve_to_tol(P, lc, args.tol, args.maxiter, check=chk, b64=b64, verify=acc)
+                res["learned_chain"] = {k: (round(v, 3) if k == "seconds" else v) for k, v in ch.items()
+                                        if k not in ("x", "history")}
+                res["learned_chain"].update(arch=arch, spec=list(spec),
+                                            weights=os.path.relpath(args.chain_weights, HERE),
+                                            speedup_wall=round(jac["seconds"] / ch["seconds"], 3))
+                log("chain to tol:", json.dumps(res["learned_chain"]))
         elif not args.no_learned:
             res["learned"] = {"error": "weights file %s missing" % args.weights}
         result["time_to_tol"] = res

@@ -388,11 +388,23 @@ int64_t mk_split_bf16_padded_dims(int64_t M, int64_t K, int64_t *M_pad,
 int mk_split_bf16_matrix(const double *a, int64_t M, int64_t K,
                          int64_t lda_src, void *hi, void *mid, void *lo,
                          void *stream);
+/* The same padded layout as two fp16 planes of 2^aexp * a (hi = fp16(x),
+ * lo = fp16(x - hi), x = 2^aexp * a); the caller picks aexp so that
+ * |x| < 2^14 (|aexp| <= 512). */
+int mk_split_f16_matrix(const double *a, int64_t M, int64_t K,
+                        int64_t lda_src, int aexp, void *hi, void *lo,
+                        void *stream);
 /* Batched C = A.B.  desc: 11 int64 per segment: M, K, N, A_hi, A_mid,
  * A_lo (device addresses), lda, b_off, b_ld, c_off, c_ld (element offsets
- * into the exec-time FP32 base pointers).  mode 0: three products
- * (hh + hm + mh, ~2^-18 relative per term); mode 1: six products (adds
- * hl + lh + mm, ~FP32 accuracy). */
+ * into the exec-time FP32 base pointers).  mode 0: three bf16 products
+ * (hh + hm + mh, ~2^-18 relative per term); mode 1: six bf16 products
+ * (adds hl + lh + mm, ~FP32 accuracy); mode 2: three fp16 products
+ * (hh + hl + lh of the fp16 pairs, ~FP32 accuracy), with fields 3, 4 the
+ * planes of mk_split_f16_matrix and field 5 its aexp; each column of B is
+ * scaled by a power of two that follows the column's largest magnitude
+ * before it is split (accumulators are rescaled exactly when it changes),
+ * and the scalings are undone exactly in the output.  B and C must not
+ * overlap (all modes). */
 typedef struct mk_rot_plan mk_rot_plan;
 int mk_rot_plan_create(int nseg, const int64_t *desc, int mode,
                        mk_rot_plan **plan);

@@ -2,8 +2,9 @@
 // 650 svd, :667 v.d at setup, :704 v.d and :709 vT.d in apply_noise).
 //
 //  Hot path (apply_noise, every A.x): C = A.B with A = v or vT (ndet x ndet)
-//  and B a TOD (ndet x n, FP32), on MFMA as split BF16, accumulated in FP32
-//  by v_mfma_f32_32x32x16_bf16.  A and B are cut into bf16 pieces
+//  and B a TOD (ndet x n, FP32), on MFMA from split operands, accumulated in
+//  FP32.  The default, mode 2, splits both into fp16 pairs (rot_f16_kernel
+//  below).  Modes 0 and 1 use v_mfma_f32_32x32x16_bf16: A and B are cut into bf16 pieces
 //  (h = RNE bf16 of x, m = bf16 of x-h, l = bf16 of x-h-m; for FP32 x the
 //  three pieces are exact).  Mode 0 keeps the three products of order <= 1,
 //  Ah.Bh + Ah.Bm + Am.Bh (about 2^-18 relative per product); mode 1 keeps
@@ -233,6 +234,214 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
     }
 }
 
+// Mode 2: FP16 pairs.  x = h + l with h = RNE fp16 of x and l = fp16 of
+// x - h represents an FP32 value to about 2^-22 relative, so the three
+// products Al.Bh + Ah.Bl + Ah.Bh (v_mfma_f32_32x32x16_f16) give about FP32
+// accuracy with half the MFMAs and two thirds of the LDS traffic of mode 1.
+// FP16's range is narrow, so both operands are scaled by powers of two
+// before the split: A (the rotation matrix) by a per-segment exponent fixed
+// when it is split (its largest entry below 2^14), and each column of B by a
+// running exponent.  The first K step sets a column's exponent from that
+// step's largest magnitude (scaled into [2^13, 2^14)); a later step that
+// would reach 2^15 lowers it (again below 2^14) and the column's
+// accumulators are multiplied by the same power of two before that step's
+// products, which is exact.  The epilogue undoes both scalings exactly
+// (ldexp), so the result depends on them only through rounding, column by
+// column as FP32 arithmetic would.  An infinity or NaN in a column reaches
+// that column's outputs only.
+//
+// Staging and tiling are those of rot_bf16_kernel except that lanes 2c and
+// 2c+1 of a wave stage column c of the wave's 32 (rows 0-7 and 16-23, 8-15
+// and 24-31 of a step), so a column's step maximum is one lane exchange.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+constexpr int kF16Top = 14;  // scaled magnitudes stay below 2^14, rescaled at 2^15 (fp16 max ~2^16)
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) rot_f16_kernel(const RotSeg* __restrict__ segs, int nseg, int64_t ntiles,
+                                                       const float* __restrict__ Bbase, float* __restrict__ Cbase) {
+  __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][BK + KPAD];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][BN][BK + KPAD];
+  __shared__ int cdel[BN];
+  __shared__ int cexp[BN];
+
+  const int64_t G = gridDim.x;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t logical = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (logical >= ntiles) return;
+  const int s = find_rseg(segs, nseg, logical);
+  const RotSeg sg = segs[s];
+  const int64_t local = logical - sg.base;
+  const int mb = (int)(local % sg.mblocks), nb = (int)(local / sg.mblocks);
+  const int m0 = mb * BM, n0 = nb * BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 1, wn = wave & 1;
+  const int K = sg.K;
+
+  const uint32_t ld = (uint32_t)sg.b_ld;
+  const __amdgpu_buffer_rsrc_t brs = buffer_rsrc(Bbase + sg.b_off, (uint32_t)K * ld * 4u);
+  const uint32_t abytes = (uint32_t)(sg.mblocks * BM) * (uint32_t)sg.lda * 2u;
+  const __amdgpu_buffer_rsrc_t ars0 = buffer_rsrc(sg.a_hi, abytes);
+  const __amdgpu_buffer_rsrc_t ars1 = buffer_rsrc(sg.a_mid, abytes);
+
+  const int nn = tid >> 1, kg0 = tid & 1;
+  const int gn = min(n0 + nn, sg.N - 1);
+  const uint32_t bvo = ((uint32_t)(kg0 * 8) * ld + (uint32_t)gn) * 4u;
+  const int arow = tid >> 1, ahalf = tid & 1;
+  const uint32_t avo = ((uint32_t)(m0 + arow) * (uint32_t)sg.lda + (uint32_t)(ahalf * 16)) * 2u;
+
+  float xv[2][8];
+  u32x4 av[2][2];
+  auto load_stage = [&](int k0) {
+    av[0][0] = __builtin_amdgcn_raw_buffer_load_b128(ars0, avo, (uint32_t)k0 * 2u, 0);
+    av[0][1] = __builtin_amdgcn_raw_buffer_load_b128(ars0, avo + 16u, (uint32_t)k0 * 2u, 0);
+    av[1][0] = __builtin_amdgcn_raw_buffer_load_b128(ars1, avo, (uint32_t)k0 * 2u, 0);
+    av[1][1] = __builtin_amdgcn_raw_buffer_load_b128(ars1, avo + 16u, (uint32_t)k0 * 2u, 0);
+    if (k0 + BK <= K) {
+#pragma unroll
+      for (int v = 0; v < 2; v++)
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+          xv[v][kk] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(brs, bvo, (uint32_t)(k0 + 16 * v + kk) * ld * 4u, 0));
+    } else {
+#pragma unroll
+      for (int v = 0; v < 2; v++)
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          const int k = k0 + kg0 * 8 + 16 * v + kk;
+          const uint32_t off = ((uint32_t)min(k, K - 1) * ld + (uint32_t)gn) * 4u;
+          const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(brs, off, 0u, 0));
+          xv[v][kk] = k < K ? x : 0.f;
+        }
+    }
+  };
+  // Column exponent: set from the first step (scaled maximum in [2^13, 2^14)),
+  // lowered when a later step would reach 2^15, again to below 2^14.
+  int sx = 0;
+  auto store_stage = [&](bool first) {
+    *reinterpret_cast<u32x4*>(&As[0][arow][ahalf * 16]) = av[0][0];
+    *reinterpret_cast<u32x4*>(&As[0][arow][ahalf * 16 + 8]) = av[0][1];
+    *reinterpret_cast<u32x4*>(&As[1][arow][ahalf * 16]) = av[1][0];
+    *reinterpret_cast<u32x4*>(&As[1][arow][ahalf * 16 + 8]) = av[1][1];
+    float m = 0.f;
+#pragma unroll
+    for (int v = 0; v < 2; v++)
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) m = fmaxf(m, fabsf(xv[v][kk]));
+    m = fmaxf(m, __shfl_xor(m, 1));
+    const int E = max((int)((__builtin_bit_cast(uint32_t, m) >> 23) & 0xff), 1);
+    int del = 0;
+    if (first) {
+      sx = kF16Top + 126 - E;
+    } else if (kF16Top + 1 + 126 - E < sx) {
+      del = kF16Top + 126 - E - sx;
+      sx += del;
+    }
+    if (kg0 == 0) {
+      cdel[nn] = del;
+      cexp[nn] = sx;
+    }
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      f16x8 ph, pl;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        const float xs = ldexpf(xv[v][kk], sx);
+        const _Float16 h = (_Float16)xs;
+        ph[kk] = h;
+        pl[kk] = (_Float16)(xs - (float)h);
+      }
+      const int kg = kg0 + 2 * v;
+      *reinterpret_cast<f16x8*>(&Bs[0][nn][kg * 8]) = ph;
+      *reinterpret_cast<f16x8*>(&Bs[1][nn][kg * 8]) = pl;
+    }
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+  const bool live0 = m0 + wm * 64 < sg.M, live1 = m0 + wm * 64 + 32 < sg.M;
+
+  load_stage(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store_stage(k0 == 0);
+    __syncthreads();
+    if (k0 + BK < K) load_stage(k0 + BK);
+    if (k0 > 0) {
+      const int d0 = cdel[wn * 64 + (lane & 31)], d1 = cdel[wn * 64 + 32 + (lane & 31)];
+      if (__builtin_amdgcn_read_exec() & __ballot(d0 != 0 || d1 != 0)) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            acc[i][0][r] = ldexpf(acc[i][0][r], d0);
+            acc[i][1][r] = ldexpf(acc[i][1][r], d1);
+          }
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kq = ks + 8 * (lane >> 5);
+      f16x8 a[2][2], b[2][2];
+#pragma unroll
+      for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const int r = wm * 64 + i * 32 + (lane & 31);
+          a[pl][i] = *reinterpret_cast<const f16x8*>(&As[pl][r][kq]);
+          const int c = wn * 64 + i * 32 + (lane & 31);
+          b[pl][i] = *reinterpret_cast<const f16x8*>(&Bs[pl][c][kq]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          if (!(i == 0 ? live0 : live1)) continue;
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  float* C = Cbase + sg.c_off;
+  const uint32_t cld = (uint32_t)sg.c_ld;
+  const int aexp = sg.pad_;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int cl = wn * 64 + j * 32 + (lane & 31);
+      const int col = n0 + cl;
+      if (col >= sg.N) continue;
+      const int shift = -(aexp + cexp[cl]);
+      const uint32_t cvo = (uint32_t)(4 * (lane >> 5)) * cld + (uint32_t)col;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int row_u = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + (r & 3);
+        if (row_u + 4 * (lane >> 5) < sg.M) C[(uint32_t)row_u * cld + cvo] = ldexpf(acc[i][j][r], shift);
+      }
+    }
+}
+
+// hi/lo fp16 planes of 2^aexp times an FP64 matrix, padded row-major.
+__global__ void split_matrix_f16_kernel(const double* __restrict__ a, int64_t M, int64_t K, int64_t lda_src,
+                                        int aexp, _Float16* __restrict__ hi, _Float16* __restrict__ lo,
+                                        int64_t M_pad, int64_t lda) {
+  int64_t n = M_pad * lda;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = i / lda, c = i - r * lda;
+    const double x = (r < M && c < K) ? ldexp(a[r * lda_src + c], aexp) : 0.0;
+    const _Float16 h = (_Float16)(float)x;
+    hi[i] = h;
+    lo[i] = (_Float16)(float)(x - (double)(float)h);
+  }
+}
+
 // hi/mid/lo split of an FP64 matrix into padded row-major bf16 planes.
 __global__ void split_matrix_kernel(const double* __restrict__ a, int64_t M, int64_t K, int64_t lda_src,
                                     __bf16* __restrict__ hi, __bf16* __restrict__ mid, __bf16* __restrict__ lo,
@@ -380,12 +589,26 @@ extern "C" int mk_split_bf16_matrix(const double* a, int64_t M, int64_t K, int64
   return mk::check_launch("split_matrix_kernel");
 }
 
-// desc per segment (11 int64): M, K, N, a_hi, a_mid, a_lo (device addresses), lda, b_off, b_ld, c_off, c_ld
+extern "C" int mk_split_f16_matrix(const double* a, int64_t M, int64_t K, int64_t lda_src, int aexp, void* hi,
+                                   void* lo, void* stream) {
+  MK_REQUIRE(a && hi && lo && M > 0 && K > 0 && lda_src >= K, MK_ERR_ARG, "bad arguments");
+  MK_REQUIRE(aexp >= -512 && aexp <= 512, MK_ERR_ARG, "bad exponent %d", aexp);
+  int64_t M_pad, lda;
+  mk_split_bf16_padded_dims(M, K, &M_pad, &lda);
+  split_matrix_f16_kernel<<<mk::grid_for(M_pad * lda, 256), 256, 0, mk::as_stream(stream)>>>(
+      a, M, K, lda_src, aexp, (_Float16*)hi, (_Float16*)lo, M_pad, lda);
+  return mk::check_launch("split_matrix_f16_kernel");
+}
+
+// desc per segment (11 int64): M, K, N, a_hi, a_mid, a_lo (device addresses), lda, b_off, b_ld, c_off, c_ld.
+// Mode 2 reads a_hi, a_lo of mk_split_f16_matrix from fields 3 and 4 and the
+// matrix's exponent from field 5.
 extern "C" int mk_rot_plan_create(int nseg, const int64_t* desc, int mode, mk_rot_plan** out) {
   MK_REQUIRE(out, MK_ERR_ARG, "out is NULL");
   *out = nullptr;
   MK_REQUIRE(nseg >= 0 && (nseg == 0 || desc), MK_ERR_ARG, "bad segment table");
-  MK_REQUIRE(mode == 0 || mode == 1, MK_ERR_ARG, "mode must be 0 (3 products) or 1 (6 products)");
+  MK_REQUIRE(mode == 0 || mode == 1 || mode == 2, MK_ERR_ARG,
+             "mode must be 0 (3 bf16 products), 1 (6 bf16 products) or 2 (3 fp16 products)");
   std::vector<RotSeg> segs(nseg);
   int64_t base = 0;
   for (int i = 0; i < nseg; i++) {
@@ -394,7 +617,13 @@ extern "C" int mk_rot_plan_create(int nseg, const int64_t* desc, int mode, mk_ro
     s.M = (int32_t)d[0]; s.K = (int32_t)d[1]; s.N = (int32_t)d[2];
     s.a_hi = (const __bf16*)(uintptr_t)d[3];
     s.a_mid = (const __bf16*)(uintptr_t)d[4];
-    s.a_lo = (const __bf16*)(uintptr_t)d[5];
+    if (mode == 2) {
+      MK_REQUIRE(d[5] >= -512 && d[5] <= 512, MK_ERR_ARG, "segment %d: bad A exponent %lld", i, (long long)d[5]);
+      s.a_lo = s.a_mid;
+      s.pad_ = (int32_t)d[5];
+    } else {
+      s.a_lo = (const __bf16*)(uintptr_t)d[5];
+    }
     s.lda = d[6];
     s.b_off = d[7]; s.b_ld = d[8]; s.c_off = d[9]; s.c_ld = d[10];
     MK_REQUIRE(d[0] > 0 && d[1] > 0 && d[2] >= 0 && d[0] < (1 << 20) && d[1] < (1 << 20) && d[2] < (1 << 30),
@@ -438,6 +667,10 @@ extern "C" int mk_rot_plan_exec(const mk_rot_plan* p, const float* b_base, float
   MK_REQUIRE(p && b_base && c_base, MK_ERR_ARG, "null argument");
   if (p->ntiles == 0) return MK_OK;
   int64_t grid = mk::cdiv(p->ntiles, 8) * 8;
+  if (p->mode == 2) {
+    rot_f16_kernel<<<(unsigned)grid, 256, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, p->ntiles, b_base, c_base);
+    return mk::check_launch("rot_f16_kernel");
+  }
   if (p->mode == 1)
     rot_bf16_kernel<6><<<(unsigned)grid, 256, 0, mk::as_stream(stream)>>>(p->d_segs, p->nseg, p->ntiles, b_base,
                                                                           c_base);

@@ -102,6 +102,7 @@ _SIGS = {
     "mk_filt_plan_destroy": ([P], I),
     "mk_split_bf16_padded_dims": ([I64, I64, P, P], I64),
     "mk_split_bf16_matrix": ([P, I64, I64, I64, P, P, P, P], I),
+    "mk_split_f16_matrix": ([P, I64, I64, I64, I, P, P, P], I),
     "mk_rot_plan_create": ([I, P, I, ctypes.POINTER(P)], I),
     "mk_rot_plan_exec": ([P, P, P, P], I),
     "mk_rot_plan_destroy": ([P], I),

@@ -184,38 +184,78 @@ class FiltPlan(_Handle):
         return out.reshape(-1, 4)[: len(self.segs)]
 
 
-class SplitMatrix:
-    """hi/mid/lo bf16 planes of an FP64 matrix, padded for the MFMA rotation."""
+# Rotation arithmetic: "f16" (mode 2, three fp16-pair products) unless
+# MK_ROT_KIND=bf16 asks for the split-BF16 modes.
+ROT_KIND = os.environ.get("MK_ROT_KIND", "f16")
 
-    def __init__(self, a):
+
+class SplitMatrix:
+    """An FP64 matrix cut into planes for the MFMA rotation, padded to the
+    kernel's tiles.  kind "f16": hi/lo fp16 planes of 2^aexp * a, with aexp
+    chosen so that every entry lies below 2^14 (mode 2); kind "bf16":
+    hi/mid/lo bf16 planes (modes 0 and 1)."""
+
+    def __init__(self, a, kind=None):
         a = np.ascontiguousarray(a, dtype=np.float64)
+        self.kind = kind or ROT_KIND
+        if self.kind not in ("f16", "bf16"):
+            raise ValueError("kind must be 'f16' or 'bf16', not %r" % (self.kind,))
         M, K = a.shape
         mp, ld = ctypes.c_int64(), ctypes.c_int64()
         mkgpu.fn("mk_split_bf16_padded_dims")(M, K, ctypes.byref(mp), ctypes.byref(ld))
         self.M, self.K, self.M_pad, self.lda = M, K, mp.value, ld.value
         d = dev.device()
         a_dev = torch.from_numpy(a).to(d)
-        self.hi = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
-        self.mid = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
-        self.lo = torch.empty(self.M_pad * self.lda, dtype=torch.bfloat16, device=d)
-        mkgpu.check("mk_split_bf16_matrix", a_dev.data_ptr(), M, K, K, self.hi.data_ptr(), self.mid.data_ptr(),
-                    self.lo.data_ptr(), dev.stream_ptr())
+        size = self.M_pad * self.lda
+        if self.kind == "f16":
+            amax = float(np.max(np.abs(a))) if a.size else 0.0
+            if not np.isfinite(amax):
+                raise ValueError("rotation matrix has non-finite entries")
+            self.aexp = 14 - int(np.frexp(amax)[1]) if amax > 0 else 0   # amax < 2^(14 - aexp)
+            self.hi = torch.empty(size, dtype=torch.float16, device=d)
+            self.lo = torch.empty(size, dtype=torch.float16, device=d)
+            mkgpu.check("mk_split_f16_matrix", a_dev.data_ptr(), M, K, K, self.aexp, self.hi.data_ptr(),
+                        self.lo.data_ptr(), dev.stream_ptr())
+        else:
+            self.aexp = 0
+            self.hi = torch.empty(size, dtype=torch.bfloat16, device=d)
+            self.mid = torch.empty(size, dtype=torch.bfloat16, device=d)
+            self.lo = torch.empty(size, dtype=torch.bfloat16, device=d)
+            mkgpu.check("mk_split_bf16_matrix", a_dev.data_ptr(), M, K, K, self.hi.data_ptr(), self.mid.data_ptr(),
+                        self.lo.data_ptr(), dev.stream_ptr())
         self._src = a_dev  # kept until the split kernel has run on the stream
+
+    def desc_planes(self):
+        """Fields 3-5 of the plan's segment descriptor."""
+        if self.kind == "f16":
+            return [self.hi.data_ptr(), self.lo.data_ptr(), self.aexp]
+        return [self.hi.data_ptr(), self.mid.data_ptr(), self.lo.data_ptr()]
 
 
 class RotPlan(_Handle):
-    """Batched split-BF16 rotation C = A.B over segments."""
+    """Batched MFMA rotation C = A.B over segments."""
 
     _destroy = "mk_rot_plan_destroy"
 
-    def __init__(self, segs, mode=1):
+    def __init__(self, segs, mode=None):
         """segs: list of (SplitMatrix, N, b_off, b_ld, c_off, c_ld).
-        mode 0: 3 bf16 products; mode 1: 6 products (about FP32 accuracy)."""
+        mode 0: 3 bf16 products; mode 1: 6 bf16 products; mode 2: 3 fp16
+        products (modes 1 and 2 are about FP32-accurate).  The default is 2
+        for fp16 planes and 1 for bf16 planes; every segment's planes must be
+        of the mode's kind."""
         super().__init__()
+        kinds = {sm.kind for sm, *_ in segs} or {ROT_KIND}
+        if len(kinds) != 1:
+            raise ValueError("segments mix fp16 and bf16 planes")
+        kind = kinds.pop()
+        if mode is None:
+            mode = 2 if kind == "f16" else 1
+        if (int(mode) == 2) != (kind == "f16"):
+            raise ValueError("mode %d does not take %s planes" % (int(mode), kind))
         rows = []
         for sm, N, b_off, b_ld, c_off, c_ld in segs:
-            rows.append([sm.M, sm.K, int(N), sm.hi.data_ptr(), sm.mid.data_ptr(), sm.lo.data_ptr(), sm.lda,
-                         int(b_off), int(b_ld), int(c_off), int(c_ld)])
+            rows.append([sm.M, sm.K, int(N)] + sm.desc_planes() + [sm.lda, int(b_off), int(b_ld), int(c_off),
+                                                                  int(c_ld)])
             self._keep.append(sm)
         desc = np.asarray(rows, dtype=np.int64).reshape(-1)
         if desc.size == 0:

@@ -28,16 +28,58 @@ def test_split_bf16_rotation_accuracy(env, ndet, n):
     rng = np.random.default_rng(ndet + n)
     v = _orth(rng, ndet)
     d = (rng.standard_normal((ndet, n)) + 3 * np.cumsum(rng.standard_normal(n))[None, :] / np.sqrt(n)).astype(np.float32)
-    sm = SplitMatrix(v)
     dt = device.to_device(d)
     ref = v @ d.astype(np.float64)
-    for mode, tol in ((0, 2e-5), (1, 1e-6)):
-        plan = RotPlan([(sm, n, 0, n, 0, n)], mode=mode)
+    for mode, kind, tol in ((0, "bf16", 2e-5), (1, "bf16", 1e-6), (2, "f16", 1e-6)):
+        plan = RotPlan([(SplitMatrix(v, kind=kind), n, 0, n, 0, n)], mode=mode)
         out = torch.empty_like(dt)
         plan.exec(dt, out)
-        e = rel_err(out.cpu().numpy(), ref)
-        print("split-bf16 rotation mode", mode, ndet, n, "rel", e, "max", max_rel_err(out.cpu().numpy(), ref))
-        assert e < tol
+        e, em = rel_err(out.cpu().numpy(), ref), max_rel_err(out.cpu().numpy(), ref)
+        print("split rotation mode", mode, ndet, n, "rel", e, "max", em)
+        assert e < tol and em < 10 * tol
+
+
+def test_f16_rotation_range(env):
+    """Mode 2 scales each column of B and the matrix by powers of two before
+    the fp16 split: columns of 1e-30 and 1e30, zero columns, a column with
+    one huge entry in the first K step or a later one, columns that grow
+    over every step, a column with an empty first step and a matrix with
+    entries far above 1 (a prewhitened v) all keep FP32-level accuracy
+    column by column, and an infinity passes through to its column only."""
+    torch, device, mkgpu = env
+    from minkasi_amd.lib.plans import RotPlan, SplitMatrix
+    rng = np.random.default_rng(5)
+    ndet, n = 200, 700
+    v = _orth(rng, ndet) * (1.0 / rng.uniform(1e-3, 1e-2, ndet))[None, :]  # entries up to ~1e3
+    d = rng.standard_normal((ndet, n))
+    colscale = 10.0 ** rng.uniform(-30, 30, n)
+    colscale[:5] = [1e-30, 1e30, 0.0, 1.0, 1e-38]
+    colscale[300:341] = 1.0
+    d = d * colscale[None, :]
+    d[7, 300] = 3e6  # one entry far above the rest of its column
+    d[150, 301] = 3e6  # ... in a later K step (the column is rescaled)
+    d[:, 302:340] *= (10.0 ** (np.arange(ndet) / 20.0))[:, None]  # rising over every step
+    d[:40, 340] = 0.0  # nothing in the first step
+    d = d.astype(np.float32)
+    ref = v @ d.astype(np.float64)
+    sm = SplitMatrix(v, kind="f16")
+    assert np.max(np.abs(v)) * 2.0 ** sm.aexp < 2 ** 14
+    plan = RotPlan([(sm, n, 0, n, 0, n)])
+    assert plan.mode == 2
+    out = torch.empty(ndet * n, dtype=torch.float32, device=device.device())
+    plan.exec(device.to_device(d), out)
+    out = out.cpu().numpy().reshape(ndet, n).astype(np.float64)
+    assert np.all(out[:, 2] == 0)
+    cols = [j for j in range(n) if j != 2]
+    err = np.array([np.linalg.norm(out[:, j] - ref[:, j]) / np.linalg.norm(ref[:, j]) for j in cols])
+    print("fp16-pair rotation over 60 decades of column scale: worst column rel err %.2e" % err.max())
+    assert err.max() < 2e-6
+    d[11, 40] = np.inf
+    o = torch.empty(ndet * n, dtype=torch.float32, device=device.device())
+    plan.exec(device.to_device(d), o)
+    o = o.cpu().numpy().reshape(ndet, n)
+    assert not np.all(np.isfinite(o[:, 40]))
+    assert np.all(np.isfinite(np.delete(o, 40, axis=1)))
 
 
 def test_rotation_batched_segments(env):
