@@ -250,9 +250,12 @@ __global__ void __launch_bounds__(256) rot_bf16_kernel(const RotSeg* __restrict_
 // column as FP32 arithmetic would.  An infinity or NaN in a column reaches
 // that column's outputs only.
 //
-// Staging and tiling are those of rot_bf16_kernel except that lanes 2c and
-// 2c+1 of a wave stage column c of the wave's 32 (rows 0-7 and 16-23, 8-15
-// and 24-31 of a step), so a column's step maximum is one lane exchange.
+// Staging and tiling are those of rot_bf16_kernel except that lanes c and
+// c+32 of wave w stage column 32w+c (rows 0-7 and 16-23, 8-15 and 24-31 of
+// a step), so a column's step maximum is one lane exchange.  A per-step
+// flag (double-buffered by step parity, so that clearing the next step's
+// flag never races with reading this one's) tells the MFMA waves whether
+// any column's exponent changed in this step.
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 constexpr int kF16Top = 14;  // scaled magnitudes stay below 2^14, rescaled at 2^15 (fp16 max ~2^16)
 
@@ -261,6 +264,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
   __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][BK + KPAD];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[2][BN][BK + KPAD];
   __shared__ int cdel[BN];
+  __shared__ int anyd[2];
   __shared__ int cexp[BN];
 
   const int64_t G = gridDim.x;
@@ -283,7 +287,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
   const __amdgpu_buffer_rsrc_t ars0 = buffer_rsrc(sg.a_hi, abytes);
   const __amdgpu_buffer_rsrc_t ars1 = buffer_rsrc(sg.a_mid, abytes);
 
-  const int nn = tid >> 1, kg0 = tid & 1;
+  const int nn = (tid >> 6) * 32 + (lane & 31), kg0 = lane >> 5;
   const int gn = min(n0 + nn, sg.N - 1);
   const uint32_t bvo = ((uint32_t)(kg0 * 8) * ld + (uint32_t)gn) * 4u;
   const int arow = tid >> 1, ahalf = tid & 1;
@@ -318,7 +322,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
   // Column exponent: set from the first step (scaled maximum in [2^13, 2^14)),
   // lowered when a later step would reach 2^15, again to below 2^14.
   int sx = 0;
-  auto store_stage = [&](bool first) {
+  auto store_stage = [&](bool first, int k0s) {
     *reinterpret_cast<u32x4*>(&As[0][arow][ahalf * 16]) = av[0][0];
     *reinterpret_cast<u32x4*>(&As[0][arow][ahalf * 16 + 8]) = av[0][1];
     *reinterpret_cast<u32x4*>(&As[1][arow][ahalf * 16]) = av[1][0];
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
     for (int v = 0; v < 2; v++)
 #pragma unroll
       for (int kk = 0; kk < 8; kk++) m = fmaxf(m, fabsf(xv[v][kk]));
-    m = fmaxf(m, __shfl_xor(m, 1));
+    m = fmaxf(m, __shfl_xor(m, 32));
     const int E = max((int)((__builtin_bit_cast(uint32_t, m) >> 23) & 0xff), 1);
     int del = 0;
     if (first) {
@@ -341,6 +345,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
       cdel[nn] = del;
       cexp[nn] = sx;
     }
+    if (tid == 0) anyd[(k0s + 1) & 1] = 0;
+    if (del != 0) anyd[k0s & 1] = 1;
 #pragma unroll
     for (int v = 0; v < 2; v++) {
       f16x8 ph, pl;
@@ -368,20 +374,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) r
 
   load_stage(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    store_stage(k0 == 0);
+    store_stage(k0 == 0, k0 / BK);
     __syncthreads();
     if (k0 + BK < K) load_stage(k0 + BK);
-    if (k0 > 0) {
+    if (k0 > 0 && anyd[(k0 / BK) & 1]) {  // some column's exponent was lowered in this step
       const int d0 = cdel[wn * 64 + (lane & 31)], d1 = cdel[wn * 64 + 32 + (lane & 31)];
-      if (__builtin_amdgcn_read_exec() & __ballot(d0 != 0 || d1 != 0)) {
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+      for (int i = 0; i < 2; i++)
 #pragma unroll
-          for (int r = 0; r < 16; r++) {
-            acc[i][0][r] = ldexpf(acc[i][0][r], d0);
-            acc[i][1][r] = ldexpf(acc[i][1][r], d1);
-          }
-      }
+        for (int r = 0; r < 16; r++) {
+          acc[i][0][r] = ldexpf(acc[i][0][r], d0);
+          acc[i][1][r] = ldexpf(acc[i][1][r], d1);
+        }
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
