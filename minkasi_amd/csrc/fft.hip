@@ -930,18 +930,34 @@ __global__ void __launch_bounds__(kThreads, 2) k1_hot_kernel(const Seg* __restri
   const float* x = in + sg.in_off + (int64_t)row * sg.in_stride;
   const int j2_0 = jb * CW;
   // real input a_i, i = 2 (P2 j1 + j2) + w, from the even extension on
-  // [-m, 2m]; reals 2 (j1 CW + c) + {0, 1} of the tile are consecutive
+  // [-m, 2m]; reals 2 (j1 CW + c) + {0, 1} of the tile are consecutive.
+  // A thread's w is the same for every u and j1 grows by kThreads / 2CW, so
+  // i is affine in u.  The extension's sample is x[m - |m - |i - m||], and
+  // past 3m that index is negative: as a 32-bit buffer offset it lies past
+  // the row's bytes, and the buffer load returns the padding's zero.
+  // (Wide tiles of short columns, 2CW > kThreads, compute i per u.)
   constexpr int NR = 2 * NPTS, NV = (NR + kThreads - 1) / kThreads;
+  const __amdgpu_buffer_rsrc_t xrs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(x), (short)0, (int)((m + 1) * 4), 0x00020000);
   float val[NV];
+  if constexpr (kThreads % (2 * CW) == 0) {
+    constexpr int DI = 2 * kHotP2 * (kThreads / (2 * CW));  // i step per u
+    const int t0 = 2 * (kHotP2 * ((int)threadIdx.x >> (LCW + 1)) + j2_0) + ((int)threadIdx.x & (2 * CW - 1)) - m;
 #pragma unroll
-  for (int u = 0; u < NV; u++) {
-    const int idx = threadIdx.x + u * kThreads;
-    const int j1 = idx >> (LCW + 1), w = idx & (2 * CW - 1);
-    const int i = 2 * (kHotP2 * j1 + j2_0) + w;
-    val[u] = 0.f;
-    if ((NV * kThreads == NR || idx < NR) && i <= 3 * m) {
-      const int t = i - m;
-      val[u] = x[t < 0 ? -t : (t <= m ? t : 2 * m - t)];
+    for (int u = 0; u < NV; u++) {
+      const int t = t0 + u * DI;
+      const int xi = m - abs(m - abs(t));
+      const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xrs, (uint32_t)xi * 4u, 0u, 0));
+      val[u] = (NV * kThreads == NR || (int)threadIdx.x + u * kThreads < NR) ? v : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < NV; u++) {
+      const int idx = threadIdx.x + u * kThreads;
+      const int t = 2 * (kHotP2 * (idx >> (LCW + 1)) + j2_0) + (idx & (2 * CW - 1)) - m;
+      const int xi = m - abs(m - abs(t));
+      const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xrs, (uint32_t)xi * 4u, 0u, 0));
+      val[u] = (NV * kThreads == NR || idx < NR) ? v : 0.f;
     }
   }
 #pragma unroll
