@@ -14,8 +14,6 @@
 //            and the value (10 B/sample for FP32).
 #include "common.h"
 
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -270,6 +268,183 @@ __global__ void __launch_bounds__(256) t2m_keys_kernel(const int32_t* __restrict
     key[i] = k;
     idx[i] = (uint32_t)i;
   }
+}
+
+// ---- Stable LSD radix sort of (uint32 key, uint32 value) pairs, 8 bits of
+// the key per pass, for the plan's sort of samples by tile.  A pass: digit
+// counts per tile of 4096 entries (sort_hist_kernel), an exclusive scan of
+// the counts in (digit, tile) order (scan_*), and a scatter that ranks the
+// entries of each tile in their input order (sort_scatter_kernel): a wave
+// finds the lanes sharing its digit with eight ballots, the four waves of a
+// round are ordered through LDS, and a per-digit running count carries the
+// order across the tile's 16 rounds.  Every count is an integer, so the
+// result does not depend on scheduling.
+constexpr int kSortThreads = 256, kSortItems = 16, kSortTile = kSortThreads * kSortItems;
+constexpr int kRadix = 256;
+
+__global__ void __launch_bounds__(kSortThreads) sort_hist_kernel(const uint32_t* __restrict__ key, int64_t n,
+                                                                 int shift, uint32_t* __restrict__ hist,
+                                                                 int64_t ntile) {
+  __shared__ uint32_t h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t t = blockIdx.x, base = t * kSortTile;
+#pragma unroll 4
+  for (int i = 0; i < kSortItems; i++) {
+    const int64_t e = base + i * kSortThreads + threadIdx.x;
+    if (e < n) atomicAdd(&h[(key[e] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntile + t] = h[threadIdx.x];
+}
+
+// Exclusive scan, step 1: each block scans 4096 entries in place and writes
+// its total.
+__global__ void __launch_bounds__(256) scan_local_kernel(uint32_t* __restrict__ a, int64_t len,
+                                                         uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sh[256];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * 4096 + tid * 16;
+  uint32_t v[16], tot = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    v[i] = base + i < len ? a[base + i] : 0u;
+    tot += v[i];
+  }
+  sh[tid] = tot;
+  __syncthreads();
+  for (int off = 1; off < 256; off *= 2) {
+    const uint32_t x = tid >= off ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  uint32_t run = sh[tid] - tot;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (base + i < len) a[base + i] = run;
+    run += v[i];
+  }
+  if (tid == 255) sums[blockIdx.x] = sh[255];
+}
+
+// Step 2 (one block): exclusive scan of the block totals, 4096 at a time.
+__global__ void __launch_bounds__(256) scan_sums_kernel(uint32_t* __restrict__ sums, int64_t nsum) {
+  __shared__ uint32_t sh[256];
+  const int tid = threadIdx.x;
+  uint32_t carry = 0;
+  for (int64_t c0 = 0; c0 < nsum; c0 += 4096) {
+    const int64_t base = c0 + tid * 16;
+    uint32_t v[16], tot = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      v[i] = base + i < nsum ? sums[base + i] : 0u;
+      tot += v[i];
+    }
+    sh[tid] = tot;
+    __syncthreads();
+    for (int off = 1; off < 256; off *= 2) {
+      const uint32_t x = tid >= off ? sh[tid - off] : 0u;
+      __syncthreads();
+      sh[tid] += x;
+      __syncthreads();
+    }
+    uint32_t run = carry + sh[tid] - tot;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (base + i < nsum) sums[base + i] = run;
+      run += v[i];
+    }
+    carry += sh[255];
+    __syncthreads();
+  }
+}
+
+// Step 3: adds each block's offset.
+__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* __restrict__ a, int64_t len,
+                                                       const uint32_t* __restrict__ sums) {
+  const uint32_t o = sums[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * 4096;
+  for (int i = threadIdx.x; i < 4096; i += 256)
+    if (base + i < len) a[base + i] += o;
+}
+
+__global__ void __launch_bounds__(kSortThreads) sort_scatter_kernel(const uint32_t* __restrict__ kin,
+                                                                    const uint32_t* __restrict__ vin, int64_t n,
+                                                                    int shift, const uint32_t* __restrict__ off,
+                                                                    int64_t ntile, uint32_t* __restrict__ kout,
+                                                                    uint32_t* __restrict__ vout) {
+  __shared__ uint32_t run[kRadix];
+  __shared__ uint32_t wcnt[kSortThreads / 64][kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t t = blockIdx.x, base = t * kSortTile;
+  run[tid] = off[(int64_t)tid * ntile + t];
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int i = 0; i < kSortItems; i++) {
+#pragma unroll
+    for (int w = 0; w < kSortThreads / 64; w++) wcnt[w][tid] = 0;  // this thread's digit column
+    __syncthreads();
+    const int64_t e = base + i * kSortThreads + tid;
+    const bool valid = e < n;
+    const uint32_t k = valid ? kin[e] : 0u, v = valid ? vin[e] : 0u;
+    const uint32_t d = (k >> shift) & (kRadix - 1);
+    uint64_t m = __ballot(valid);  // valid lanes of this wave with digit d
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bb = __ballot(valid && ((d >> b) & 1u));
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & below);
+    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / 64; w++) add += wcnt[w][tid];
+    run[tid] += add;
+  }
+}
+
+// Workspace (bytes) of sort_pairs for n entries.
+int64_t sort_pairs_work_bytes(int64_t n) {
+  const int64_t ntile = std::max<int64_t>(1, mk::cdiv(n, kSortTile));
+  const int64_t len = kRadix * ntile, nsum = mk::cdiv(len, 4096);
+  return (len + nsum) * 4 + 256;
+}
+
+// Stable sort of n pairs by key bits [0, end_bit).  (kA, vA) hold the input;
+// passes alternate between A and B, and *in_b says where the result is.
+int sort_pairs(uint32_t* kA, uint32_t* vA, uint32_t* kB, uint32_t* vB, int64_t n, int end_bit, void* work,
+               hipStream_t st, bool* in_b) {
+  *in_b = false;
+  if (n <= 0) return MK_OK;
+  const int64_t ntile = mk::cdiv(n, kSortTile);
+  const int64_t len = kRadix * ntile, nsum = mk::cdiv(len, 4096);
+  MK_REQUIRE(n < (int64_t(1) << 31) && ntile < INT32_MAX, MK_ERR_ARG, "sort: %lld entries are too many",
+             (long long)n);
+  uint32_t* hist = (uint32_t*)work;
+  uint32_t* sums = hist + len;
+  for (int shift = 0; shift < std::max(end_bit, 1); shift += 8) {
+    const uint32_t* ki = *in_b ? kB : kA;
+    const uint32_t* vi = *in_b ? vB : vA;
+    uint32_t* ko = *in_b ? kA : kB;
+    uint32_t* vo = *in_b ? vA : vB;
+    sort_hist_kernel<<<(unsigned)ntile, kSortThreads, 0, st>>>(ki, n, shift, hist, ntile);
+    scan_local_kernel<<<(unsigned)nsum, 256, 0, st>>>(hist, len, sums);
+    scan_sums_kernel<<<1, 256, 0, st>>>(sums, nsum);
+    scan_add_kernel<<<(unsigned)nsum, 256, 0, st>>>(hist, len, sums);
+    sort_scatter_kernel<<<(unsigned)ntile, kSortThreads, 0, st>>>(ki, vi, n, shift, hist, ntile, ko, vo);
+    int r = mk::check_launch("sort_scatter_kernel");
+    if (r) return r;
+    *in_b = !*in_b;
+  }
+  return MK_OK;
 }
 
 // Run lengths of the sorted tile keys without atomics: the last sample of
@@ -580,12 +755,7 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   if (keys.alloc(segmax * 4) || keys2.alloc(segmax * 4) || idx.alloc(segmax * 4) || cnt.alloc(ntiles * 8 + 8) ||
       nbad.alloc(8))
     return fail(MK_ERR_ALLOC);
-  size_t tmp_bytes = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t*)keys.p, (uint32_t*)keys2.p,
-                                         (uint32_t*)idx.p, (uint32_t*)pl->entry, (int)segmax, 0, 32,
-                                         stream) != hipSuccess)
-    return fail((mk::set_error("hipcub sort sizing failed"), MK_ERR_HIP));
-  if (tmp.alloc(tmp_bytes)) return fail(MK_ERR_ALLOC);
+  if (tmp.alloc(sort_pairs_work_bytes(segmax))) return fail(MK_ERR_ALLOC);
   int end_bit = 1;
   while ((int64_t(1) << end_bit) < ntiles) end_bit++;
   std::vector<int32_t> h_cnt(2 * ntiles);
@@ -604,10 +774,16 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
       mk::set_error("t2m plan: %lld samples have ipix outside [0, %d*%d)", (long long)h_bad, nx, ny);
       return fail(MK_ERR_RANGE);
     }
-    size_t tb = tmp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, (uint32_t*)keys.p, (uint32_t*)keys2.p, (uint32_t*)idx.p,
-                                           (uint32_t*)(pl->entry + s0), (int)n, 0, end_bit, stream) != hipSuccess)
-      return fail((mk::set_error("hipcub sort failed"), MK_ERR_HIP));
+    bool in_b = false;
+    {
+      const int r = sort_pairs((uint32_t*)keys.p, (uint32_t*)idx.p, (uint32_t*)keys2.p, pl->entry + s0, n, end_bit,
+                               tmp.p, stream, &in_b);
+      if (r) return fail(r);
+    }
+    if (!in_b &&  // an even number of passes left the result in (keys, idx)
+        (hipMemcpyAsync(keys2.p, keys.p, n * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+         hipMemcpyAsync(pl->entry + s0, idx.p, n * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess))
+      return fail((mk::set_error("t2m plan: copy of the sorted entries failed"), MK_ERR_HIP));
     t2m_count_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>((uint32_t*)keys2.p, n, (int32_t*)cnt.p,
                                                                 (int32_t*)cnt.p + ntiles);
     t2m_lpix_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix, s0, pl->entry + s0, n, ny, tile_x, tile_y,
