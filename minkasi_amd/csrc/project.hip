@@ -3,18 +3,22 @@
 //  map2tod : coalesced gather, dat[i] (+)= map[ipix[i]].  8 B/sample of HBM
 //            traffic (int32 index in, value out); the map (<= 16 MB FP32 at
 //            2048^2) stays in L2 / Infinity Cache.
-//  tod2map : deterministic plan.  Samples are stably sorted by map tile,
-//            each 64-sample group is ordered by pixel, and each work item
-//            (<= item_samples entries of one tile) reduces its samples into a
-//            wave-private LDS tile with a wave-level segmented scan (no
-//            atomics), sums the four wave tiles in a fixed order and writes a
-//            partial tile.  A second kernel sums each tile's partials in item
-//            order.  The map is therefore bitwise reproducible run to run.
-//            Per sample the exec reads a uint32 entry, a uint16 local pixel
-//            and the value (10 B/sample for FP32).
+//  tod2map : deterministic plan.  Samples are stably sorted by map tile and
+//            cut into work items (<= item_samples entries of one tile), and
+//            each item into groups of up to 64 entries in sample order made
+//            of at most four runs of consecutive samples.  A wave reads a
+//            group's values in sample order from its 20-byte header, moves
+//            them into pixel order with the group's lane words (local pixel
+//            and source lane) and reduces them into a wave-private LDS tile
+//            with a wave-level segmented scan (no atomics); the four wave
+//            tiles are summed in a fixed order into a partial tile, and a
+//            second kernel sums each tile's partials in item order.  The map
+//            is therefore bitwise reproducible run to run.  Per sample the
+//            exec reads about 4.3 bytes of plan and the value, contiguously.
 #include "common.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -228,18 +232,30 @@ extern "C" int mk_tod2map_atomic_f64(double* map, const double* dat, int64_t nsa
 
 struct T2MItem {
   int64_t seg_base;   // first sample of the segment the entries are relative to
-  int64_t begin;      // first entry (index into entry/lpix arrays)
+  int64_t begin;      // first entry of the item in the build's sorted entry array
   int32_t count;      // number of entries
   int32_t tile;       // tile id = tx * ntile_y + ty
+  int64_t g_begin;    // first group of the item
+  int32_t g_count;    // number of groups
+  int32_t pad_;
+};
+
+// Up to 64 consecutive entries of one item, in sample order, made of at
+// most four runs of consecutive samples.  Lane l of a wave takes entry l:
+// its run r is the number of run starts (start1..3) at or below l, and its
+// sample offset within the segment is base[r] + l - start_r.
+struct T2MGroup {
+  uint32_t base[4];  // sample offset of each run's first entry (unused runs: 0)
+  uint32_t starts;   // bits 0-7, 8-15, 16-23: first lane of runs 1, 2, 3 (64 = no such run); 24-30: entries
 };
 
 struct mk_t2m_plan {
   int64_t nsamp = 0;
   int32_t nx = 0, ny = 0, tile_x = 0, tile_y = 0, ntile_x = 0, ntile_y = 0;
   int32_t item_samples = 0;
-  int64_t nitems = 0, ntiles_touched = 0, max_item = 0;
-  uint32_t* entry = nullptr;     // [nsamp] sample offset within segment
-  uint16_t* lpix = nullptr;      // [nsamp] pixel within tile
+  int64_t nitems = 0, ntiles_touched = 0, max_item = 0, ngroups = 0;
+  T2MGroup* groups = nullptr;    // [ngroups]
+  uint32_t* lanes = nullptr;     // [ngroups][64]: lane d in pixel order: local pixel | source lane << 16
   T2MItem* items = nullptr;      // [nitems] sorted by (tile, segment, position)
   int32_t* tile_items = nullptr; // [ntiles+1] CSR into items
   void* partial = nullptr;       // [nitems][tile_px] partial tiles
@@ -460,58 +476,96 @@ __global__ void __launch_bounds__(256) t2m_count_kernel(const uint32_t* __restri
   }
 }
 
-// Writes local pixel per sorted entry.
-__global__ void __launch_bounds__(256) t2m_lpix_kernel(const int32_t* __restrict__ ipix, int64_t seg_base,
-                                                       const uint32_t* __restrict__ idx, int64_t n, int32_t ny,
-                                                       int32_t tx, int32_t ty, uint16_t* __restrict__ lpix) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    int32_t p = ipix[seg_base + idx[i]];
-    int32_t x = p / ny, y = p - x * ny;
-    lpix[i] = (uint16_t)((x % tx) * ty + (y % ty));
+// Group formation, one thread per item walking its entries in sample
+// order: a group closes at 64 entries, or when a fifth run would start.
+// WRITE = false counts the item's groups; WRITE = true writes their headers
+// and, for the permutation kernel, each group's first entry and segment.
+template <bool WRITE>
+__global__ void __launch_bounds__(256) t2m_groups_kernel(T2MItem* __restrict__ items, int64_t nitems,
+                                                         const uint32_t* __restrict__ entry,
+                                                         T2MGroup* __restrict__ groups,
+                                                         int64_t* __restrict__ gpos, int64_t* __restrict__ gseg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nitems) return;
+  const T2MItem it = items[i];
+  int64_t g = WRITE ? it.g_begin : 0;
+  int cnt = 0, nruns = 0;
+  uint32_t base[4] = {0, 0, 0, 0};
+  int start[4] = {0, 64, 64, 64};
+  uint32_t prev = 0;
+  int64_t first = it.begin;
+  auto close = [&]() {
+    if (WRITE) {
+      T2MGroup G;
+      for (int r = 0; r < 4; r++) G.base[r] = r < nruns ? base[r] : 0u;
+      G.starts = (uint32_t)start[1] | ((uint32_t)start[2] << 8) | ((uint32_t)start[3] << 16) | ((uint32_t)cnt << 24);
+      groups[g] = G;
+      gpos[g] = first;
+      gseg[g] = it.seg_base;
+    }
+    g++;
+  };
+  for (int64_t k = 0; k < it.count; k++) {
+    const uint32_t e = entry[it.begin + k];
+    bool newrun = cnt == 0 || e != prev + 1u;
+    if (cnt == 64 || (newrun && nruns == 4)) {
+      close();
+      cnt = 0; nruns = 0; newrun = true;
+      start[1] = start[2] = start[3] = 64;
+      first = it.begin + k;
+    }
+    if (newrun) {
+      base[nruns] = e;
+      start[nruns] = cnt;
+      nruns++;
+    }
+    cnt++;
+    prev = e;
   }
+  if (cnt > 0) close();
+  if (!WRITE) items[i].g_count = (int32_t)g;
 }
 
-// One wave per 64-entry group: order the group's entries by local pixel
-// (ties by position, so the order is unique) with a bitonic sort in
-// registers.  Groups never straddle items.
-__global__ void __launch_bounds__(256) t2m_groupsort_kernel(const T2MItem* __restrict__ items, int64_t nitems,
-                                                            const int64_t* __restrict__ group_base,
-                                                            int64_t ngroups, uint32_t* __restrict__ entry,
-                                                            uint16_t* __restrict__ lpix) {
-  int lane = threadIdx.x & 63;
-  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (g >= ngroups) return;  // whole wave exits together
-  // locate item by binary search on group_base
-  int64_t lo = 0, hi = nitems - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (group_base[mid] <= g) lo = mid; else hi = mid - 1;
+// One wave per group: the local pixel of each entry and the group's order
+// by local pixel (ties by lane, so the order is unique) from a bitonic sort
+// in registers; lane d of the result gets the local pixel and source lane of
+// the d-th entry in pixel order.
+__device__ __forceinline__ void group_lanes(const T2MGroup* __restrict__ groups, int64_t g,
+                                            const int64_t* __restrict__ gpos, const int64_t* __restrict__ gseg,
+                                            const uint32_t* __restrict__ entry, const int32_t* __restrict__ ipix,
+                                            int32_t ny, int32_t tx, int32_t ty, uint32_t* __restrict__ lanes,
+                                            int lane) {
+  const int cnt = (int)(groups[g].starts >> 24);
+  uint32_t lp = 0xFFFFu;
+  if (lane < cnt) {
+    const int32_t p = ipix[gseg[g] + entry[gpos[g] + lane]];
+    const int32_t x = p / ny, y = p - x * ny;
+    lp = (uint32_t)((x % tx) * ty + (y % ty));
   }
-  const T2MItem it = items[lo];
-  int64_t gi = g - group_base[lo];
-  int64_t pos0 = it.begin + gi * 64;
-  int64_t end = it.begin + it.count;
-  int64_t pos = pos0 + lane;
-  bool valid = pos < end;
-  uint32_t lp = valid ? lpix[pos] : 0xFFFFu;
-  uint32_t e = valid ? entry[pos] : 0u;
   uint32_t key = (lp << 6) | (uint32_t)lane;
   for (int k = 2; k <= 64; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      uint32_t other = __shfl_xor(key, j, 64);
-      bool up = ((lane & k) == 0);
-      bool lower = ((lane & j) == 0);
-      uint32_t mn = min(key, other), mx = max(key, other);
+      const uint32_t other = __shfl_xor(key, j, 64);
+      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+      const uint32_t mn = min(key, other), mx = max(key, other);
       key = (lower == up) ? mn : mx;
     }
   }
-  int src = (int)(key & 63u);
-  uint32_t e_sorted = __shfl(e, src, 64);
-  if (valid) {
-    entry[pos] = e_sorted;
-    lpix[pos] = (uint16_t)(key >> 6);
-  }
+  lanes[g * 64 + lane] = (key >> 6) | ((key & 63u) << 16);
+}
+
+// Waves stride over the groups: one dispatch holds fewer than 2^32
+// work-items, and a plan of 8e9 samples has 1.3e8 groups of 64 lanes.
+__global__ void __launch_bounds__(256) t2m_group_lanes_kernel(const T2MGroup* __restrict__ groups, int64_t ngroups,
+                                                              const int64_t* __restrict__ gpos,
+                                                              const int64_t* __restrict__ gseg,
+                                                              const uint32_t* __restrict__ entry,
+                                                              const int32_t* __restrict__ ipix, int32_t ny,
+                                                              int32_t tx, int32_t ty, uint32_t* __restrict__ lanes) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; g < ngroups; g += nwaves)
+    group_lanes(groups, g, gpos, gseg, entry, ipix, ny, tx, ty, lanes, lane);
 }
 
 // Lane-to-lane moves inside a wave through DPP (no LDS round trip).  CTRL is
@@ -550,12 +604,12 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
 }
 
 // Exec stage 1: one workgroup (4 waves) per item.  Wave w takes the item's
-// 64-entry groups w, w+4, w+8, ... in that order, kT2MUnroll of them per
-// iteration so that their loads overlap.  Within a group the entries are
-// ordered by local pixel, so every pixel is one run; the run's last lane
-// adds the run sum into the wave's private LDS tile (one add per pixel per
-// group, in group order: the result does not depend on timing).
-constexpr int kT2MUnroll = 16;
+// groups w, w+4, w+8, ... in that order, kT2MUnroll of them per iteration so
+// that their loads overlap.  Within a group the values are moved into pixel
+// order, so every pixel is one run of lanes; the run's last lane adds the
+// run sum into the wave's private LDS tile (one add per pixel per group, in
+// group order: the result does not depend on timing).
+constexpr int kT2MUnroll = 8;
 
 // Weight of polarisation component comp of mode POL (MK_POL_*) for a sample
 // at angle 2 gamma with sin s and cos c (minkasi.c:178-258: QU -> (c, s),
@@ -578,63 +632,90 @@ __device__ __forceinline__ T pol_weight(int comp, T s, T c) {
 __device__ __forceinline__ void pol_sincos(float a, float* s, float* c) { sincosf(a, s, c); }
 __device__ __forceinline__ void pol_sincos(double a, double* s, double* c) { sincos(a, s, c); }
 
+// Moves v from lane src to this lane (ds_bpermute, the LDS crossbar without
+// a memory access).
+__device__ __forceinline__ float lane_move(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_move(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // TD: sample type; T: accumulation type (FP64 for FP32 samples keeps the
 // per-pixel sums of many cancelling samples at FP64 accuracy); P: partials.
 // POL: 0 bins the samples; MK_POL_* bins sample * pol_weight(comp, 2 gamma)
-// with 2 gamma read from twogamma at the same entry.
+// with 2 gamma read from twogamma at the same sample.
+//
+// A wave reads a group's samples in sample order, lane l the group's l-th
+// entry (runs of consecutive samples, so the loads are contiguous), from the
+// group header alone; the lane word then moves each value to its place in
+// the group's pixel order, where every pixel is one run of lanes.
 template <typename TD, typename T, typename P, int POL = 0>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
-                                                        const uint32_t* __restrict__ entry,
-                                                        const uint16_t* __restrict__ lpix,
+                                                        const T2MGroup* __restrict__ groups,
+                                                        const uint32_t* __restrict__ lanes,
                                                         const TD* __restrict__ dat, int tile_px,
                                                         P* __restrict__ partial,
                                                         const TD* __restrict__ twogamma = nullptr, int comp = 0) {
   extern __shared__ unsigned char smem_raw[];
   T* tiles = reinterpret_cast<T*>(smem_raw);  // [4][tile_px]
   const T2MItem it = items[blockIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int p = threadIdx.x; p < 4 * tile_px; p += 256) tiles[p] = T(0);
   __syncthreads();
   T* mine = tiles + wave * tile_px;
   const TD* base = dat + it.seg_base;
-  const int64_t end = it.begin + it.count;
-  const int ngroups = (it.count + 63) >> 6;
+  const int ngroups = it.g_count;
   const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
   for (int g0 = wave; g0 < ngroups; g0 += 4 * kT2MUnroll) {
-    int lp[kT2MUnroll];
-    uint32_t e[kT2MUnroll];
-    bool valid[kT2MUnroll];
+    uint32_t lw[kT2MUnroll];
     T v[kT2MUnroll];
-    // Loads are unconditional (index clamped to the item's last entry, the
-    // item is never empty) so that all of them are in flight together.
+    // Every load is unconditional (groups past the item repeat its last
+    // group, lanes past a group's entries read its first sample), so the
+    // headers, then the lane words and values of all kT2MUnroll groups are
+    // in flight together; what is past the end is discarded below.  A value's
+    // address needs only the (uniform) header.
+    uint32_t st[kT2MUnroll], b0[kT2MUnroll], b1[kT2MUnroll], b2[kT2MUnroll], b3[kT2MUnroll];
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
-      const int64_t pos = it.begin + (int64_t)(g0 + 4 * j) * 64 + lane;
-      valid[j] = pos < end;  // false for every lane of a group past the item
-      const int64_t pc = valid[j] ? pos : end - 1;
-      lp[j] = (int)lpix[pc];
-      e[j] = entry[pc];
+      const T2MGroup* G = groups + it.g_begin + min(g0 + 4 * j, ngroups - 1);
+      st[j] = G->starts;
+      b0[j] = G->base[0]; b1[j] = G->base[1]; b2[j] = G->base[2]; b3[j] = G->base[3];
     }
 #pragma unroll
+    for (int j = 0; j < kT2MUnroll; j++)
+      lw[j] = lanes[(it.g_begin + min(g0 + 4 * j, ngroups - 1)) * 64 + lane];
+#pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
-      T x = (T)base[e[j]];
+      const int s1 = (int)(st[j] & 255u), s2 = (int)((st[j] >> 8) & 255u), s3 = (int)((st[j] >> 16) & 255u),
+                cnt = (int)(st[j] >> 24);
+      const uint32_t b = lane >= s3 ? b3[j] - s3 : (lane >= s2 ? b2[j] - s2 : (lane >= s1 ? b1[j] - s1 : b0[j]));
+      const bool in = lane < cnt;
+      const uint32_t e = in ? b + (uint32_t)lane : b0[j];
+      T x = (T)base[e];
       if constexpr (POL != 0) {
         TD sn, cs;
-        pol_sincos(twogamma[it.seg_base + e[j]], &sn, &cs);
-        x = (T)(base[e[j]] * pol_weight<POL, TD>(comp, sn, cs));
+        pol_sincos(twogamma[it.seg_base + e], &sn, &cs);
+        x = (T)(base[e] * pol_weight<POL, TD>(comp, sn, cs));
       }
-      v[j] = valid[j] ? x : T(0);
-      lp[j] = valid[j] ? lp[j] : -1 - lane;  // distinct sentinels never match
+      v[j] = in ? x : T(0);
     }
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
       if (g0 + 4 * j >= ngroups) break;  // wave-uniform
-      const int prev = __shfl_up(lp[j], 1, 64);
-      const uint64_t heads = __ballot(lane == 0 || prev != lp[j]);
+      const int lpv = (int)(lw[j] & 0xFFFFu);
+      const bool valid = lpv != 0xFFFF;
+      const T x = lane_move(v[j], (int)((lw[j] >> 16) & 63u));
+      const int lp = valid ? lpv : -1 - lane;  // distinct sentinels never match
+      const int prev = __shfl_up(lp, 1, 64);
+      const uint64_t heads = __ballot(lane == 0 || prev != lp);
       const int start = 63 - __clzll(heads & upto);
-      const T s = wave_segmented_sum(v[j], lane, lane - start);
-      const bool tail = valid[j] && (lane == 63 || ((heads >> lane) >> 1) & 1ull);
-      if (tail) __hip_atomic_fetch_add(mine + lp[j], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const T sum = wave_segmented_sum(valid ? x : T(0), lane, lane - start);
+      const bool tail = valid && (lane == 63 || ((heads >> lane) >> 1) & 1ull);
+      if (tail) __hip_atomic_fetch_add(mine + lpv, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
   __syncthreads();
@@ -715,8 +796,8 @@ struct DevBuf {
 
 void free_plan(mk_t2m_plan* pl) {
   if (!pl) return;
-  if (pl->entry) (void)hipFree(pl->entry);
-  if (pl->lpix) (void)hipFree(pl->lpix);
+  if (pl->groups) (void)hipFree(pl->groups);
+  if (pl->lanes) (void)hipFree(pl->lanes);
   if (pl->items) (void)hipFree(pl->items);
   if (pl->tile_items) (void)hipFree(pl->tile_items);
   if (pl->partial) (void)hipFree(pl->partial);
@@ -744,11 +825,9 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   const int tile_px = tile_x * tile_y;
   auto fail = [&](int code) { free_plan(pl); return code; };
 
-  if (hipMalloc((void**)&pl->entry, std::max<int64_t>(nsamp, 1) * 4) != hipSuccess ||
-      hipMalloc((void**)&pl->lpix, std::max<int64_t>(nsamp, 1) * 2) != hipSuccess) {
-    mk::set_error("t2m plan: hipMalloc of %lld entries failed", (long long)nsamp);
-    return fail(MK_ERR_ALLOC);
-  }
+  DevBuf entry_b;  // the samples sorted by tile, offsets within their segment (build only)
+  if (entry_b.alloc(std::max<int64_t>(nsamp, 1) * 4)) return fail(MK_ERR_ALLOC);
+  uint32_t* const entry = (uint32_t*)entry_b.p;
   std::vector<T2MItem> h_items;
   const int64_t segmax = std::min<int64_t>(kSegment, std::max<int64_t>(nsamp, 1));
   DevBuf keys, keys2, idx, cnt, nbad, tmp;
@@ -776,19 +855,17 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
     }
     bool in_b = false;
     {
-      const int r = sort_pairs((uint32_t*)keys.p, (uint32_t*)idx.p, (uint32_t*)keys2.p, pl->entry + s0, n, end_bit,
+      const int r = sort_pairs((uint32_t*)keys.p, (uint32_t*)idx.p, (uint32_t*)keys2.p, entry + s0, n, end_bit,
                                tmp.p, stream, &in_b);
       if (r) return fail(r);
     }
     if (!in_b &&  // an even number of passes left the result in (keys, idx)
         (hipMemcpyAsync(keys2.p, keys.p, n * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
-         hipMemcpyAsync(pl->entry + s0, idx.p, n * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess))
+         hipMemcpyAsync(entry + s0, idx.p, n * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess))
       return fail((mk::set_error("t2m plan: copy of the sorted entries failed"), MK_ERR_HIP));
     t2m_count_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>((uint32_t*)keys2.p, n, (int32_t*)cnt.p,
                                                                 (int32_t*)cnt.p + ntiles);
-    t2m_lpix_kernel<<<mk::grid_for(n, 256), 256, 0, stream>>>(ipix, s0, pl->entry + s0, n, ny, tile_x, tile_y,
-                                                               pl->lpix + s0);
-    if (mk::check_launch("t2m_lpix_kernel")) return fail(MK_ERR_HIP);
+    if (mk::check_launch("t2m_count_kernel")) return fail(MK_ERR_HIP);
     if (hipMemcpyAsync(h_cnt.data(), cnt.p, ntiles * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
       return fail((mk::set_error("t2m plan: count copy failed"), MK_ERR_HIP));
@@ -796,7 +873,7 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
     for (int64_t t = 0; t < ntiles; t++) {
       int64_t c = (int64_t)h_cnt[ntiles + t] - h_cnt[t];
       for (int64_t o = 0; o < c; o += item_samples) {
-        T2MItem it;
+        T2MItem it{};
         it.seg_base = s0;
         it.begin = pos + o;
         it.count = (int32_t)std::min<int64_t>(item_samples, c - o);
@@ -810,12 +887,8 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   std::stable_sort(h_items.begin(), h_items.end(), [](const T2MItem& a, const T2MItem& b) { return a.tile < b.tile; });
   pl->nitems = (int64_t)h_items.size();
   std::vector<int32_t> h_tile_items(ntiles + 1, 0);
-  std::vector<int64_t> h_group_base(std::max<int64_t>(pl->nitems, 1), 0);
-  int64_t ngroups = 0;
   for (int64_t i = 0; i < pl->nitems; i++) {
     h_tile_items[h_items[i].tile + 1]++;
-    h_group_base[i] = ngroups;
-    ngroups += (h_items[i].count + 63) / 64;
     pl->max_item = std::max<int64_t>(pl->max_item, h_items[i].count);
   }
   for (int64_t t = 0; t < ntiles; t++) {
@@ -826,16 +899,38 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   if (hipMalloc((void**)&pl->items, std::max<int64_t>(pl->nitems, 1) * sizeof(T2MItem)) != hipSuccess ||
       hipMalloc((void**)&pl->tile_items, (ntiles + 1) * 4) != hipSuccess)
     return fail((mk::set_error("t2m plan: item alloc failed"), MK_ERR_ALLOC));
-  DevBuf gb;
-  if (gb.alloc(h_group_base.size() * 8)) return fail(MK_ERR_ALLOC);
   if (hipMemcpyAsync(pl->items, h_items.data(), pl->nitems * sizeof(T2MItem), hipMemcpyHostToDevice, stream) ||
-      hipMemcpyAsync(pl->tile_items, h_tile_items.data(), (ntiles + 1) * 4, hipMemcpyHostToDevice, stream) ||
-      hipMemcpyAsync(gb.p, h_group_base.data(), h_group_base.size() * 8, hipMemcpyHostToDevice, stream))
+      hipMemcpyAsync(pl->tile_items, h_tile_items.data(), (ntiles + 1) * 4, hipMemcpyHostToDevice, stream))
     return fail((mk::set_error("t2m plan: upload failed"), MK_ERR_HIP));
-  if (ngroups > 0) {
-    t2m_groupsort_kernel<<<(unsigned)mk::cdiv(ngroups * 64, 256), 256, 0, stream>>>(
-        pl->items, pl->nitems, (const int64_t*)gb.p, ngroups, pl->entry, pl->lpix);
-    if (mk::check_launch("t2m_groupsort_kernel")) return fail(MK_ERR_HIP);
+  // groups: count per item, then the headers, then each group's pixel order
+  if (pl->nitems > 0) {
+    const unsigned gi = (unsigned)mk::cdiv(pl->nitems, 256);
+    t2m_groups_kernel<false><<<gi, 256, 0, stream>>>(pl->items, pl->nitems, entry, nullptr, nullptr, nullptr);
+    if (mk::check_launch("t2m_groups_kernel (count)")) return fail(MK_ERR_HIP);
+    if (hipMemcpyAsync(h_items.data(), pl->items, pl->nitems * sizeof(T2MItem), hipMemcpyDeviceToHost, stream) ||
+        hipStreamSynchronize(stream))
+      return fail((mk::set_error("t2m plan: group count copy failed"), MK_ERR_HIP));
+    for (int64_t i = 0; i < pl->nitems; i++) {
+      h_items[i].g_begin = pl->ngroups;
+      pl->ngroups += h_items[i].g_count;
+    }
+    if (hipMemcpyAsync(pl->items, h_items.data(), pl->nitems * sizeof(T2MItem), hipMemcpyHostToDevice, stream))
+      return fail((mk::set_error("t2m plan: upload failed"), MK_ERR_HIP));
+    DevBuf gpos, gseg;
+    if (hipMalloc((void**)&pl->groups, std::max<int64_t>(pl->ngroups, 1) * sizeof(T2MGroup)) != hipSuccess ||
+        hipMalloc((void**)&pl->lanes, std::max<int64_t>(pl->ngroups, 1) * 64 * 4) != hipSuccess ||
+        gpos.alloc(std::max<int64_t>(pl->ngroups, 1) * 8) || gseg.alloc(std::max<int64_t>(pl->ngroups, 1) * 8))
+      return fail((mk::set_error("t2m plan: group alloc (%lld groups) failed", (long long)pl->ngroups),
+                   MK_ERR_ALLOC));
+    t2m_groups_kernel<true><<<gi, 256, 0, stream>>>(pl->items, pl->nitems, entry, pl->groups, (int64_t*)gpos.p,
+                                                    (int64_t*)gseg.p);
+    if (mk::check_launch("t2m_groups_kernel (write)")) return fail(MK_ERR_HIP);
+    t2m_group_lanes_kernel<<<mk::grid_for(pl->ngroups * 64, 256, 256 * 256), 256, 0, stream>>>(
+        pl->groups, pl->ngroups, (const int64_t*)gpos.p, (const int64_t*)gseg.p, entry, ipix, ny, tile_x, tile_y,
+        pl->lanes);
+    if (mk::check_launch("t2m_group_lanes_kernel")) return fail(MK_ERR_HIP);
+    if (hipStreamSynchronize(stream) != hipSuccess)  // before the temporaries go
+      return fail((mk::set_error("t2m plan: sync failed"), MK_ERR_HIP));
   }
   // partial tiles: 8-byte elements when affordable (FP64 exec), else 4
   int64_t pb8 = std::max<int64_t>(pl->nitems, 1) * tile_px * 8;
@@ -844,7 +939,8 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   if (hipMalloc(&pl->partial, pl->partial_bytes) != hipSuccess)
     return fail((mk::set_error("t2m plan: partial alloc (%lld B) failed", (long long)pl->partial_bytes), MK_ERR_ALLOC));
   if (hipStreamSynchronize(stream) != hipSuccess) return fail((mk::set_error("t2m plan: sync failed"), MK_ERR_HIP));
-  pl->plan_bytes = nsamp * 6 + pl->nitems * (int64_t)sizeof(T2MItem) + (ntiles + 1) * 4;
+  pl->plan_bytes = pl->ngroups * (int64_t)(sizeof(T2MGroup) + 64 * 4) + pl->nitems * (int64_t)sizeof(T2MItem) +
+                   (ntiles + 1) * 4;
   *out = pl;
   return MK_OK;
 }
@@ -879,10 +975,10 @@ int t2m_exec(const mk_t2m_plan* pl, TM* map, const TD* dat, int accumulate, void
   MK_REQUIRE(p64 || sizeof(TD) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
   if (pl->nitems > 0) {
     if (p64)
-      t2m_items_kernel<TD, T, double><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix,
+      t2m_items_kernel<TD, T, double><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->groups, pl->lanes,
                                                                                   dat, tile_px, (double*)pl->partial);
     else
-      t2m_items_kernel<TD, T, float><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->entry, pl->lpix,
+      t2m_items_kernel<TD, T, float><<<(unsigned)pl->nitems, 256, lds, stream>>>(pl->items, pl->groups, pl->lanes,
                                                                                  dat, tile_px, (float*)pl->partial);
     int st = mk::check_launch("t2m_items_kernel");
     if (st) return st;
@@ -946,10 +1042,10 @@ int t2m_exec_pol_mode(const mk_t2m_plan* pl, TD* map, const TD* dat, const TD* t
     if (pl->nitems > 0) {
       if (p64)
         t2m_items_kernel<TD, T, double, POL><<<(unsigned)pl->nitems, 256, lds, stream>>>(
-            pl->items, pl->entry, pl->lpix, dat, tile_px, (double*)pl->partial, twogamma, c);
+            pl->items, pl->groups, pl->lanes, dat, tile_px, (double*)pl->partial, twogamma, c);
       else
         t2m_items_kernel<TD, T, float, POL><<<(unsigned)pl->nitems, 256, lds, stream>>>(
-            pl->items, pl->entry, pl->lpix, dat, tile_px, (float*)pl->partial, twogamma, c);
+            pl->items, pl->groups, pl->lanes, dat, tile_px, (float*)pl->partial, twogamma, c);
       int st = mk::check_launch("t2m_items_kernel (pol)");
       if (st) return st;
     }

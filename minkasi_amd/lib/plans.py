@@ -40,7 +40,7 @@ class T2MPlan(_Handle):
 
     _destroy = "mk_t2m_plan_destroy"
 
-    def __init__(self, ipix, nx, ny, tile=(64, 32), item_samples=None):
+    def __init__(self, ipix, nx, ny, tile=(32, 32), item_samples=None):
         super().__init__()
         n = int(ipix.numel())
         if item_samples is None:

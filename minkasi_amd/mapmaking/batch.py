@@ -98,7 +98,7 @@ class BatchedDot:
                 return False
         return True
 
-    def __init__(self, todvec, skymap, tile=(64, 32), item_samples=None, tiled_gather=None, filter_work_cap=None):
+    def __init__(self, todvec, skymap, tile=(32, 32), item_samples=None, tiled_gather=None, filter_work_cap=None):
         """tiled_gather: gather map2tod from a tiled copy of the map through a
         second index array (4 bytes per sample), or from the row-major map
         (False); None decides from the free device memory.
@@ -142,7 +142,7 @@ class BatchedDot:
         tuner = tune.active()
         # the tod2map plan first: its build needs sort buffers over 2^30-sample
         # segments on top of the 6 bytes per sample it keeps
-        if tuner is not None and off > 0 and tile == (64, 32) and item_samples is None:
+        if tuner is not None and off > 0 and tile == (32, 32) and item_samples is None:
             self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
         else:
             self.t2m = T2MPlan(self.ipix[:off], self.nx, self.ny, tile=tile, item_samples=item_samples)

@@ -152,6 +152,23 @@ def test_config2_full_operator_and_grouped_filter():
     assert len(op.filts) > 1
     _operator_properties(P, "config 2")
     n = op.nsamp
+    # tod2map over all 8.2e9 samples against torch's index_add in FP64: the
+    # plan's build runs one wave per group of 64 samples, more groups than
+    # one dispatch can hold at a wave each
+    a = op.buf_a[:n]
+    a.normal_(generator=torch.Generator(device="cuda").manual_seed(3))
+    ipix = op.ipix[:n]
+    ref = torch.zeros(P.map.nx * P.map.ny, dtype=torch.float64, device="cuda")
+    for c0 in range(0, n, 1 << 26):
+        c1 = min(n, c0 + (1 << 26))
+        ref.index_add_(0, ipix[c0:c1].long(), a[c0:c1].double())
+    got = torch.zeros(P.map.nx * P.map.ny, dtype=torch.float32, device="cuda")
+    op.t2m.exec(got, a, False, acc64=False)
+    d = got.double() - ref
+    e, em = float(d.norm() / ref.norm()), float(d.abs().max() / ref.abs().max())
+    print("config 2 tod2map against index_add: rel %.2e, max %.2e" % (e, em))
+    assert e < 2e-6 and em < 1e-5
+    del ref, got, d
     b = op.buf_b[:n]
     for tod, o, (ndet, nn) in zip(op.tods, op.offs, op.sizes):
         b[o:o + ndet * nn].copy_(tod.get_data().reshape(-1))
