@@ -249,13 +249,17 @@ struct T2MGroup {
   uint32_t starts;   // bits 0-7, 8-15, 16-23: first lane of runs 1, 2, 3 (64 = no such run); 24-30: entries
 };
 
+// A group's lane words are 16 bits: the local pixel in bits 0-9 and the
+// source lane in bits 10-15, so a tile holds at most 1024 pixels.
+constexpr int kT2MMaxTilePx = 1024;
+
 struct mk_t2m_plan {
   int64_t nsamp = 0;
   int32_t nx = 0, ny = 0, tile_x = 0, tile_y = 0, ntile_x = 0, ntile_y = 0;
   int32_t item_samples = 0;
   int64_t nitems = 0, ntiles_touched = 0, max_item = 0, ngroups = 0;
   T2MGroup* groups = nullptr;    // [ngroups]
-  uint32_t* lanes = nullptr;     // [ngroups][64]: lane d in pixel order: local pixel | source lane << 16
+  uint16_t* lanes = nullptr;     // [ngroups][64]: lane d in pixel order: local pixel | source lane << 10
   T2MItem* items = nullptr;      // [nitems] sorted by (tile, segment, position)
   int32_t* tile_items = nullptr; // [ntiles+1] CSR into items
   void* partial = nullptr;       // [nitems][tile_px] partial tiles
@@ -533,7 +537,7 @@ __global__ void __launch_bounds__(256) t2m_groups_kernel(T2MItem* __restrict__ i
 __device__ __forceinline__ void group_lanes(const T2MGroup* __restrict__ groups, int64_t g,
                                             const int64_t* __restrict__ gpos, const int64_t* __restrict__ gseg,
                                             const uint32_t* __restrict__ entry, const int32_t* __restrict__ ipix,
-                                            int32_t ny, int32_t tx, int32_t ty, uint32_t* __restrict__ lanes,
+                                            int32_t ny, int32_t tx, int32_t ty, uint16_t* __restrict__ lanes,
                                             int lane) {
   const int cnt = (int)(groups[g].starts >> 24);
   uint32_t lp = 0xFFFFu;
@@ -551,7 +555,8 @@ __device__ __forceinline__ void group_lanes(const T2MGroup* __restrict__ groups,
       key = (lower == up) ? mn : mx;
     }
   }
-  lanes[g * 64 + lane] = (key >> 6) | ((key & 63u) << 16);
+  // lanes at or past the group's count sort last and stay unread
+  lanes[g * 64 + lane] = (uint16_t)(((key >> 6) & (kT2MMaxTilePx - 1)) | ((key & 63u) << 10));
 }
 
 // Waves stride over the groups: one dispatch holds fewer than 2^32
@@ -561,7 +566,7 @@ __global__ void __launch_bounds__(256) t2m_group_lanes_kernel(const T2MGroup* __
                                                               const int64_t* __restrict__ gseg,
                                                               const uint32_t* __restrict__ entry,
                                                               const int32_t* __restrict__ ipix, int32_t ny,
-                                                              int32_t tx, int32_t ty, uint32_t* __restrict__ lanes) {
+                                                              int32_t tx, int32_t ty, uint16_t* __restrict__ lanes) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; g < ngroups; g += nwaves)
@@ -605,11 +610,12 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int lane, int d) {
 
 // Exec stage 1: one workgroup (4 waves) per item.  Wave w takes the item's
 // groups w, w+4, w+8, ... in that order, kT2MUnroll of them per iteration so
-// that their loads overlap.  Within a group the values are moved into pixel
-// order, so every pixel is one run of lanes; the run's last lane adds the
-// run sum into the wave's private LDS tile (one add per pixel per group, in
-// group order: the result does not depend on timing).
-constexpr int kT2MUnroll = 8;
+// that their loads overlap, and loads the next iteration's group headers
+// before it reduces the current groups.  Within a group the values are moved
+// into pixel order, so every pixel is one run of lanes; the run's last lane
+// adds the run sum into the wave's private LDS tile (one add per pixel per
+// group, in group order: the result does not depend on timing).
+constexpr int kT2MUnroll = 4;
 
 // Weight of polarisation component comp of mode POL (MK_POL_*) for a sample
 // at angle 2 gamma with sin s and cos c (minkasi.c:178-258: QU -> (c, s),
@@ -644,6 +650,26 @@ __device__ __forceinline__ double lane_move(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// A group header as the kernel uses it: the starts word and, per run r,
+// base[r] - start_r, so that lane l of run r reads sample o_r + l.
+struct T2MHdr {
+  uint32_t st, o0, o1, o2, o3;
+};
+__device__ __forceinline__ T2MHdr t2m_load_hdr(const T2MGroup* G) {
+  const uint4 b = *reinterpret_cast<const uint4*>(G->base);
+  const uint32_t st = G->starts;
+  return {st, b.x, b.y - (st & 255u), b.z - ((st >> 8) & 255u), b.w - ((st >> 16) & 255u)};
+}
+// Sample offset of lane l without branches: the run starts are ordered
+// (unused runs start at 64), so adding each run's step from the previous one
+// for every start at or below l telescopes to o_r.
+__device__ __forceinline__ uint32_t t2m_lane_sample(const T2MHdr& h, int l) {
+  const uint32_t m1 = 0u - (uint32_t)(l >= (int)(h.st & 255u));
+  const uint32_t m2 = 0u - (uint32_t)(l >= (int)((h.st >> 8) & 255u));
+  const uint32_t m3 = 0u - (uint32_t)(l >= (int)((h.st >> 16) & 255u));
+  return h.o0 + (m1 & (h.o1 - h.o0)) + (m2 & (h.o2 - h.o1)) + (m3 & (h.o3 - h.o2)) + (uint32_t)l;
+}
+
 // TD: sample type; T: accumulation type (FP64 for FP32 samples keeps the
 // per-pixel sums of many cancelling samples at FP64 accuracy); P: partials.
 // POL: 0 bins the samples; MK_POL_* bins sample * pol_weight(comp, 2 gamma)
@@ -656,7 +682,7 @@ __device__ __forceinline__ double lane_move(double v, int src) {
 template <typename TD, typename T, typename P, int POL = 0>
 __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restrict__ items,
                                                         const T2MGroup* __restrict__ groups,
-                                                        const uint32_t* __restrict__ lanes,
+                                                        const uint16_t* __restrict__ lanes,
                                                         const TD* __restrict__ dat, int tile_px,
                                                         P* __restrict__ partial,
                                                         const TD* __restrict__ twogamma = nullptr, int comp = 0) {
@@ -670,45 +696,40 @@ __global__ void __launch_bounds__(256) t2m_items_kernel(const T2MItem* __restric
   const TD* base = dat + it.seg_base;
   const int ngroups = it.g_count;
   const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
-  for (int g0 = wave; g0 < ngroups; g0 += 4 * kT2MUnroll) {
-    uint32_t lw[kT2MUnroll];
-    T v[kT2MUnroll];
-    // Every load is unconditional (groups past the item repeat its last
-    // group, lanes past a group's entries read its first sample), so the
-    // headers, then the lane words and values of all kT2MUnroll groups are
-    // in flight together; what is past the end is discarded below.  A value's
-    // address needs only the (uniform) header.
-    uint32_t st[kT2MUnroll], b0[kT2MUnroll], b1[kT2MUnroll], b2[kT2MUnroll], b3[kT2MUnroll];
+  const T2MGroup* G0 = groups + it.g_begin;
+  // Every load is unconditional: groups past the item repeat its last group
+  // (discarded below) and lanes past a group's entries reread its last entry.
+  T2MHdr h[kT2MUnroll];
 #pragma unroll
-    for (int j = 0; j < kT2MUnroll; j++) {
-      const T2MGroup* G = groups + it.g_begin + min(g0 + 4 * j, ngroups - 1);
-      st[j] = G->starts;
-      b0[j] = G->base[0]; b1[j] = G->base[1]; b2[j] = G->base[2]; b3[j] = G->base[3];
-    }
+  for (int j = 0; j < kT2MUnroll; j++) h[j] = t2m_load_hdr(G0 + min(wave + 4 * j, ngroups - 1));
+  for (int g0 = wave; g0 < ngroups; g0 += 4 * kT2MUnroll) {
+    uint32_t lw[kT2MUnroll], cnt[kT2MUnroll];
+    T v[kT2MUnroll];
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++)
       lw[j] = lanes[(it.g_begin + min(g0 + 4 * j, ngroups - 1)) * 64 + lane];
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
-      const int s1 = (int)(st[j] & 255u), s2 = (int)((st[j] >> 8) & 255u), s3 = (int)((st[j] >> 16) & 255u),
-                cnt = (int)(st[j] >> 24);
-      const uint32_t b = lane >= s3 ? b3[j] - s3 : (lane >= s2 ? b2[j] - s2 : (lane >= s1 ? b1[j] - s1 : b0[j]));
-      const bool in = lane < cnt;
-      const uint32_t e = in ? b + (uint32_t)lane : b0[j];
+      cnt[j] = h[j].st >> 24;
+      const uint32_t e = t2m_lane_sample(h[j], min(lane, (int)cnt[j] - 1));
       T x = (T)base[e];
       if constexpr (POL != 0) {
         TD sn, cs;
         pol_sincos(twogamma[it.seg_base + e], &sn, &cs);
         x = (T)(base[e] * pol_weight<POL, TD>(comp, sn, cs));
       }
-      v[j] = in ? x : T(0);
+      v[j] = lane < (int)cnt[j] ? x : T(0);
     }
+    // the next iteration's headers, in flight while these groups reduce
+#pragma unroll
+    for (int j = 0; j < kT2MUnroll; j++)
+      h[j] = t2m_load_hdr(G0 + min(g0 + 4 * kT2MUnroll + 4 * j, ngroups - 1));
 #pragma unroll
     for (int j = 0; j < kT2MUnroll; j++) {
       if (g0 + 4 * j >= ngroups) break;  // wave-uniform
-      const int lpv = (int)(lw[j] & 0xFFFFu);
-      const bool valid = lpv != 0xFFFF;
-      const T x = lane_move(v[j], (int)((lw[j] >> 16) & 63u));
+      const int lpv = (int)(lw[j] & (kT2MMaxTilePx - 1));
+      const bool valid = lane < (int)cnt[j];  // pixel-order lanes past the count are the padding
+      const T x = lane_move(v[j], (int)(lw[j] >> 10));
       const int lp = valid ? lpv : -1 - lane;  // distinct sentinels never match
       const int prev = __shfl_up(lp, 1, 64);
       const uint64_t heads = __ballot(lane == 0 || prev != lp);
@@ -812,8 +833,8 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   *out = nullptr;
   MK_REQUIRE(nsamp >= 0 && (nsamp == 0 || ipix), MK_ERR_ARG, "bad ipix/nsamp");
   MK_REQUIRE(nx > 0 && ny > 0 && (int64_t)nx * ny <= INT32_MAX, MK_ERR_ARG, "bad map size");
-  MK_REQUIRE(tile_x > 0 && tile_y > 0 && tile_x * tile_y <= 8192 && tile_y % 4 == 0, MK_ERR_ARG,
-             "tile must have 4..8192 pixels with tile_y a multiple of 4 (got %d x %d)", tile_x, tile_y);
+  MK_REQUIRE(tile_x > 0 && tile_y > 0 && tile_x * tile_y <= kT2MMaxTilePx && tile_y % 4 == 0, MK_ERR_ARG,
+             "tile must have 4..%d pixels with tile_y a multiple of 4 (got %d x %d)", kT2MMaxTilePx, tile_x, tile_y);
   MK_REQUIRE(item_samples >= 64 && item_samples % 64 == 0, MK_ERR_ARG, "item_samples must be a positive multiple of 64");
   hipStream_t stream = mk::as_stream(stream_v);
   mk_t2m_plan* pl = new mk_t2m_plan();
@@ -918,7 +939,7 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
       return fail((mk::set_error("t2m plan: upload failed"), MK_ERR_HIP));
     DevBuf gpos, gseg;
     if (hipMalloc((void**)&pl->groups, std::max<int64_t>(pl->ngroups, 1) * sizeof(T2MGroup)) != hipSuccess ||
-        hipMalloc((void**)&pl->lanes, std::max<int64_t>(pl->ngroups, 1) * 64 * 4) != hipSuccess ||
+        hipMalloc((void**)&pl->lanes, std::max<int64_t>(pl->ngroups, 1) * 64 * 2) != hipSuccess ||
         gpos.alloc(std::max<int64_t>(pl->ngroups, 1) * 8) || gseg.alloc(std::max<int64_t>(pl->ngroups, 1) * 8))
       return fail((mk::set_error("t2m plan: group alloc (%lld groups) failed", (long long)pl->ngroups),
                    MK_ERR_ALLOC));
@@ -939,7 +960,7 @@ extern "C" int mk_t2m_plan_create(const int32_t* ipix, int64_t nsamp, int32_t nx
   if (hipMalloc(&pl->partial, pl->partial_bytes) != hipSuccess)
     return fail((mk::set_error("t2m plan: partial alloc (%lld B) failed", (long long)pl->partial_bytes), MK_ERR_ALLOC));
   if (hipStreamSynchronize(stream) != hipSuccess) return fail((mk::set_error("t2m plan: sync failed"), MK_ERR_HIP));
-  pl->plan_bytes = pl->ngroups * (int64_t)(sizeof(T2MGroup) + 64 * 4) + pl->nitems * (int64_t)sizeof(T2MItem) +
+  pl->plan_bytes = pl->ngroups * (int64_t)(sizeof(T2MGroup) + 64 * 2) + pl->nitems * (int64_t)sizeof(T2MItem) +
                    (ntiles + 1) * 4;
   *out = pl;
   return MK_OK;
@@ -954,20 +975,7 @@ int t2m_exec(const mk_t2m_plan* pl, TM* map, const TD* dat, int accumulate, void
   hipStream_t stream = mk::as_stream(stream_v);
   const int tile_px = pl->tile_x * pl->tile_y;
   const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
-  const size_t lds = 4 * (size_t)tile_px * sizeof(T);
-  // four wave tiles of the accumulation type; above 64 KiB (FP64 sums of
-  // tiles over 2048 pixels) the kernel is allowed the rest of gfx950's LDS
-  constexpr size_t kMaxT2MLds = 160 * 1024;
-  MK_REQUIRE(lds <= kMaxT2MLds, MK_ERR_ARG, "tile of %d pixels too large for LDS with this precision", tile_px);
-  if (lds > 65536) {
-    static std::once_flag once;
-    std::call_once(once, [] {
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, double>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, float>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
-    });
-  }
+  const size_t lds = 4 * (size_t)tile_px * sizeof(T);  // four wave tiles, at most 32 KiB
   // FP64 partial tiles whenever the plan holds them (8-byte partial
   // elements); an FP64 accumulation with 4-byte partials rounds each item's
   // sum once
@@ -1022,20 +1030,9 @@ int t2m_exec_pol_mode(const mk_t2m_plan* pl, TD* map, const TD* dat, const TD* t
   const int ncomp = POL == MK_POL_QU ? 2 : (POL == MK_POL_IQU_PRECON ? 6 : 3);
   const int tile_px = pl->tile_x * pl->tile_y;
   const int64_t ntiles = (int64_t)pl->ntile_x * pl->ntile_y;
-  const size_t lds = 4 * (size_t)tile_px * sizeof(T);
-  constexpr size_t kMaxT2MLds = 160 * 1024;
-  MK_REQUIRE(lds <= kMaxT2MLds, MK_ERR_ARG, "tile of %d pixels too large for LDS with FP64 sums", tile_px);
+  const size_t lds = 4 * (size_t)tile_px * sizeof(T);  // four wave tiles, at most 32 KiB
   const bool p64 = pl->partial_elem_bytes == 8;
   MK_REQUIRE(p64 || sizeof(TD) == 4, MK_ERR_ARG, "plan too large for FP64 partial tiles; use FP32");
-  if (lds > 65536) {
-    static std::once_flag once;
-    std::call_once(once, [] {
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, double, POL>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(t2m_items_kernel<TD, T, float, POL>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxT2MLds);
-    });
-  }
   const int bpt = (int)mk::cdiv(tile_px / 4, 256);
   const unsigned nblk = (unsigned)(ntiles * bpt);
   for (int c = 0; c < ncomp; c++) {

@@ -33,7 +33,7 @@ import numpy as np
 
 _active = None
 
-T2M_TILES = ((32, 32), (64, 32), (32, 64), (64, 64), (32, 16))
+T2M_TILES = ((32, 32), (32, 16), (64, 16), (16, 32), (16, 16))
 HOT_P2 = 2048
 
 

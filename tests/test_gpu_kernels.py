@@ -157,11 +157,12 @@ def _plan(env, ipix_t, nx, ny, tx=64, ty=32, item=4096):
     return h
 
 
-@pytest.mark.parametrize("dtype,tile", [("float32", (64, 32)), ("float64", (64, 32)), ("float32", (32, 32)),
-                                        ("float64", (32, 32))])
+@pytest.mark.parametrize("dtype,tile", [("float32", (64, 16)), ("float64", (64, 16)), ("float32", (32, 32)),
+                                        ("float64", (32, 32)), ("float32", (32, 16))])
 def test_tod2map_plan_matches_oracle_and_is_deterministic(env, dtype, tile):
-    """Two tile shapes: the default 64 x 32 and the 32 x 32 the autotuner picks
-    on the benchmark workload."""
+    """Tiles of 1024 pixels (the most a plan takes: a lane word keeps a
+    10-bit local pixel) in two shapes, and the 32 x 16 the autotuner picks on
+    the benchmark workload."""
     torch, device, mkgpu = env
     rng = np.random.default_rng(4)
     ra_b, dec_b, dra, ddec = raster_pointing(rng, 64, 65536)
@@ -213,7 +214,7 @@ def test_tod2map_plan_edge_cases(env):
     ref = np.bincount(ipix.ravel(), tod.ravel(), minlength=nx * ny)
     ipt = device.to_device(ipix)
     tdev = device.to_device(tod)
-    for tx, ty, item in [(8, 8, 64), (16, 4, 128), (64, 32, 4096), (32, 32, 192), (4, 4, 64)]:
+    for tx, ty, item in [(8, 8, 64), (16, 4, 128), (64, 16, 4096), (32, 32, 192), (4, 4, 64)]:
         h = _plan(env, ipt, nx, ny, tx, ty, item)
         try:
             mt = torch.zeros(nx * ny, dtype=torch.float64, device=ipt.device)
@@ -230,6 +231,10 @@ def test_tod2map_plan_edge_cases(env):
     st = mkgpu.fn("mk_t2m_plan_create")(badt.data_ptr(), bad.size, nx, ny, 8, 8, 64, ctypes.byref(h),
                                          device.stream_ptr())
     assert st == mkgpu.MK_ERR_RANGE
+    # a tile over 1024 pixels is refused
+    st = mkgpu.fn("mk_t2m_plan_create")(ipt.data_ptr(), ipix.size, nx, ny, 64, 32, 4096, ctypes.byref(h),
+                                         device.stream_ptr())
+    assert st == mkgpu.MK_ERR_ARG
     # empty input: map cleared (accumulate=0) or untouched (accumulate=1)
     empty = torch.zeros(0, dtype=torch.int32, device=ipt.device)
     h = _plan(env, empty, nx, ny, 8, 8, 64)
