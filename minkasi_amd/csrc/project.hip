@@ -656,9 +656,8 @@ struct T2MHdr {
   uint32_t st, o0, o1, o2, o3;
 };
 __device__ __forceinline__ T2MHdr t2m_load_hdr(const T2MGroup* G) {
-  const uint4 b = *reinterpret_cast<const uint4*>(G->base);
   const uint32_t st = G->starts;
-  return {st, b.x, b.y - (st & 255u), b.z - ((st >> 8) & 255u), b.w - ((st >> 16) & 255u)};
+  return {st, G->base[0], G->base[1] - (st & 255u), G->base[2] - ((st >> 8) & 255u), G->base[3] - ((st >> 16) & 255u)};
 }
 // Sample offset of lane l without branches: the run starts are ordered
 // (unused runs start at 64), so adding each run's step from the previous one

@@ -148,7 +148,7 @@ def test_map2tod_misaligned_views_bit_exact(env, dtype):
             assert np.array_equal(got, expect.astype(dtype)), (oi, od, do_add)
 
 
-def _plan(env, ipix_t, nx, ny, tx=64, ty=32, item=4096):
+def _plan(env, ipix_t, nx, ny, tx=32, ty=32, item=4096):
     torch, device, mkgpu = env
     import ctypes
     h = ctypes.c_void_p()
@@ -200,6 +200,43 @@ def test_tod2map_plan_matches_oracle_and_is_deterministic(env, dtype, tile):
         assert np.array_equal(mt.cpu().numpy(), hits.astype(dtype))
     finally:
         mkgpu.check("mk_t2m_plan_destroy", h)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 16, 63, 64, 100])
+def test_tod2map_plan_run_coded_groups(env, k):
+    """Samples alternate between two map tiles in blocks of k, so each tile's
+    stable-sorted entries form runs of k consecutive samples: groups close at
+    the fifth run (4k entries when 4k < 64), at 64 entries, or with runs that
+    cross group boundaries (k >= 64).  FP64 sums equal np.bincount to
+    summation-order rounding, FP32 sums within 1e-5, both bitwise
+    repeatable; every case also has a partial last group."""
+    torch, device, mkgpu = env
+    rng = np.random.default_rng(100 + k)
+    nx, ny, tile = 64, 64, 16
+    n = 3 * 4096 + 77
+    blk = (np.arange(n) // k) % 2
+    # tile (0, 0) or tile (1, 2), a random pixel inside it
+    x = np.where(blk == 0, 0, 16) + rng.integers(0, tile, n)
+    y = np.where(blk == 0, 0, 32) + rng.integers(0, tile, n)
+    ipix = (x * ny + y).astype(np.int32)
+    tod = rng.standard_normal(n)
+    ref = np.bincount(ipix, tod, minlength=nx * ny)
+    ipt = device.to_device(ipix)
+    for item in (64, 4096):
+        h = _plan(env, ipt, nx, ny, tile, tile, item)
+        try:
+            for dtype, tol in (("float64", 1e-12), ("float32", 1e-5)):
+                td = device.to_device(tod.astype(dtype))
+                name = "mk_t2m_plan_exec_f64" if dtype == "float64" else "mk_t2m_plan_exec_f32"
+                outs = []
+                for _ in range(2):
+                    mt = torch.full((nx * ny,), 3.0, dtype=getattr(torch, dtype), device=td.device)
+                    mkgpu.check(name, h, mt.data_ptr(), td.data_ptr(), 0, device.stream_ptr())
+                    outs.append(mt.cpu().numpy())
+                assert np.array_equal(outs[0], outs[1])
+                assert np.abs(outs[0] - ref).max() <= tol * np.abs(ref).max(), (k, item, dtype)
+        finally:
+            mkgpu.check("mk_t2m_plan_destroy", h)
 
 
 def test_tod2map_plan_edge_cases(env):
