@@ -162,7 +162,8 @@ int mk_tod2map_atomic_f64(double *map, const double *dat, int64_t nsamp,
  * (stable), every 64-sample group is ordered by pixel, and the work is cut
  * into items.  Executing the plan sums each pixel's samples in a fixed
  * order, so the map is bitwise reproducible.  tile_y must be a multiple of
- * 4 and tile_x * tile_y at most 1024.  The plan owns device memory
+ * 4 and tile_x * tile_y at most 1024; item_samples (samples of one tile per
+ * workgroup) must be a positive multiple of 64.  The plan owns device memory
  * allocated at creation; exec allocates nothing. */
 typedef struct mk_t2m_plan mk_t2m_plan;
 int mk_t2m_plan_create(const int32_t *ipix, int64_t nsamp, int32_t nx,
