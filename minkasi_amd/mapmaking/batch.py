@@ -141,7 +141,8 @@ class BatchedDot:
         from .. import tune
         tuner = tune.active()
         # the tod2map plan first: its build needs sort buffers over 2^30-sample
-        # segments on top of the 6 bytes per sample it keeps
+        # segments and a 4-byte sorted entry per sample on top of the 2.3 bytes
+        # per sample it keeps
         if tuner is not None and off > 0 and tile == (32, 32) and item_samples is None:
             self.t2m = tuner.t2m_plan(self.ipix[:off], self.nx, self.ny, dat=self.buf_a[:off])
         else:

@@ -228,8 +228,9 @@ class Tuner:
             e1.synchronize()
             return e0.elapsed_time(e1) / reps
 
-        # one candidate plan alive at a time (a plan holds 6 bytes per
-        # sample); the winner is built again at the end
+        # one candidate plan alive at a time (a plan holds about 2.3 bytes
+        # per sample, its build several times that); the winner is built
+        # again at the end
         best, best_ms, results = None, math.inf, []
         for tile in T2M_TILES:
             plan = T2MPlan(ipix, nx, ny, tile=tile)

@@ -89,3 +89,12 @@ def test_flat_namespace_and_single_rank_names():
     assert (mk.myrank, mk.nproc, mk.have_mpi, mk.comm) == (0, 1, False, None)
     assert mk.get_nthread() >= 1 and mk.set_nthread(4) is None
     mk.barrier()  # no device work queued, no GPU needed
+
+
+def test_t2m_tile_candidates_fit_the_plan():
+    """Every tod2map tile the autotuner may pick is one the plan accepts: at
+    most 1024 pixels (a lane word keeps a 10-bit local pixel) and tile_y a
+    multiple of 4 (the per-tile sum reads 4 pixels of one tile row)."""
+    from minkasi_amd import tune
+    for tx, ty in tune.T2M_TILES:
+        assert 4 <= tx * ty <= 1024 and ty % 4 == 0, (tx, ty)
