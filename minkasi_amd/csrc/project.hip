@@ -10,16 +10,16 @@
 //            group's values in sample order from its 20-byte header, moves
 //            them into pixel order with the group's lane words (local pixel
 //            and source lane) and reduces them into a wave-private LDS tile
-//            with a wave-level segmented scan (no atomics); the four wave
-//            tiles are summed in a fixed order into a partial tile, and a
-//            second kernel sums each tile's partials in item order.  The map
-//            is therefore bitwise reproducible run to run.  Per sample the
-//            exec reads about 4.3 bytes of plan and the value, contiguously.
+//            with a wave-level segmented scan and one LDS add per pixel run
+//            (no global atomics); the four wave tiles are summed in a fixed
+//            order into a partial tile, and a second kernel sums each tile's
+//            partials in item order.  The map is therefore bitwise
+//            reproducible run to run.  Per sample the exec reads about 2.3
+//            bytes of plan and the value, contiguously.
 #include "common.h"
 
 #include <algorithm>
 #include <cstdlib>
-#include <mutex>
 #include <vector>
 
 namespace {
